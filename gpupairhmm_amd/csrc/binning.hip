@@ -1,0 +1,352 @@
+// Device-side preparation of a batch: base encoding, per-read bins and a
+// counting sort of all alignments into a work list ordered by (bin,
+// descending hap length).
+//
+// Replaces the reference's convert_DNA / partitionIndicesKernel /
+// computeAlignmentsPerPartitionPerBatch plus its thrust/cub scans and the
+// per-chunk host round trip (utility_kernels.cuh:6-102; gpuworker.cuh:747-888).
+// The reference partitions reads by the smallest tile that holds them; here a
+// bin is (tile class, arithmetic form), and alignments inside a bin are
+// ordered by hap length so the groups of one wave sweep the same number of
+// columns.  Within a record, every alignment of a hap shares the hap's
+// length, so counts and positions are reserved per (hap, bin) and the
+// atomics scale with haps x bins, not alignments.
+//
+// Views passed here are self-contained: reads_prefix[0] == 0,
+// haps_prefix[0] == 0, and read/hap offsets index read_data/hap_data.
+#include "phmm_device.hpp"
+
+namespace phmm {
+
+
+
+__global__ void k_clear(int32_t* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ void k_encode_reads(BinParams bp)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bp.read_bytes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)base_code(bp.read_data[i]);
+        const uint32_t q = min((uint32_t)bp.bq[i], 127u);
+        const uint32_t qi = min((uint32_t)bp.iq[i], 127u);
+        const uint32_t qd = min((uint32_t)bp.dq[i], 127u);
+        bp.rows[i] = b | (q << 8) | (qi << 16) | (qd << 24);
+    }
+}
+
+__global__ void k_encode_haps(BinParams bp)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bp.hap_bytes;
+         i += (int64_t)gridDim.x * blockDim.x)
+        bp.hapc[i] = (uint8_t)base_code(bp.hap_data[i]);
+}
+
+__global__ void k_hap_max(BinParams bp)
+{
+    int m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < bp.nH; i += gridDim.x * blockDim.x)
+        m = max(m, bp.hap_len[i]);
+    for (int o = 32; o > 0; o >>= 1)
+        m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(bp.hmax, m);
+}
+
+// Per read: its bin.  Tile class k is the smallest tile with room for the
+// read plus one out-of-bounds row; reads beyond the largest tile go to the
+// multi-tile bin 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when
+// the read's coefficients keep it in range, else bin 2k+1 (SAFE).
+__global__ void k_read_meta(BinParams bp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.nR)
+        return;
+    const int L = bp.read_len[i];
+    ReadMeta m;
+    m.nl = 1.0f;
+    if (L <= 0) {
+        m.bin = -1;
+        bp.meta[i] = m;
+        return;
+    }
+    int k = bp.ntiles;
+    for (int j = 0; j < bp.ntiles; ++j)
+        if (L + 1 <= bp.tiles[j]) {
+            k = j;
+            break;
+        }
+    if (k == bp.ntiles) {
+        m.bin = 2 * bp.ntiles;
+        bp.meta[i] = m;
+        return;
+    }
+    const int off = bp.read_off[i];
+    float eps = kEpsGpu;
+    if (bp.semantics == 1)
+        eps = bp.ph2pr[min((int)bp.gcp[off], 127)];
+    bool ok = bp.allow_fast != 0 && eps <= 0.5f;
+    double nl = 1.0;
+    float pdelta = 0.f, palpha = 1.f;
+    for (int r = 0; r < L && ok; ++r) {
+        const uint32_t w = bp.rows[off + r];
+        const float delta = bp.ph2pr[row_iq(w)];
+        const float sigma = bp.ph2pr[row_dq(w)];
+        const float alpha = 1.0f - (delta + sigma);
+        if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
+            ok = false;
+        else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
+            ok = false;
+        nl *= (double)alpha;
+        pdelta = delta;
+        palpha = alpha;
+    }
+    if (ok && !(nl >= 1e-3))
+        ok = false;
+    m.nl = ok ? (float)nl : 1.0f;
+    m.bin = 2 * k + (ok ? 0 : 1);
+    bp.meta[i] = m;
+}
+
+// Exclusive prefix of R_g*H_g (the reference's resultOffsetsPerBatch,
+// gpuworker.cuh:802-815), one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
+{
+    __shared__ int64_t wsum[16];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = 0; base < bp.G; base += 1024) {
+        const int g = base + threadIdx.x;
+        int64_t v = g < bp.G ? (int64_t)bp.nreads[g] * bp.nhaps[g] : 0;
+        int64_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(x, o, 64);
+            if (lane >= o)
+                x += y;
+        }
+        if (lane == 63)
+            wsum[wv] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t s = 0;
+            for (int k = 0; k < 16; ++k) {
+                const int64_t t = wsum[k];
+                wsum[k] = s;
+                s += t;
+            }
+        }
+        __syncthreads();
+        const int64_t incl = x + wsum[wv] + carry;
+        if (g < bp.G)
+            bp.align_off[g] = incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023)
+            carry = incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        bp.align_off[bp.G] = carry;
+}
+
+// One block per group: count its reads per bin, lay them out bin by bin.
+__global__ __launch_bounds__(256) void k_group_bins(BinParams bp)
+{
+    __shared__ int cnt[2 * kMaxTiles + 2];
+    __shared__ int pos[2 * kMaxTiles + 2];
+    const int g = blockIdx.x;
+    const int NB1 = bp.NB + 1;  // slot NB: empty reads
+    for (int b = threadIdx.x; b < NB1; b += blockDim.x)
+        cnt[b] = 0;
+    __syncthreads();
+    const int r0 = bp.rprefix[g], nr = bp.nreads[g];
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+        const int b = bp.meta[r0 + r].bin;
+        atomicAdd(&cnt[b < 0 ? bp.NB : b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int b = 0; b < NB1; ++b) {
+            pos[b] = s;
+            bp.gb_start[(int64_t)g * NB1 + b] = s;
+            bp.gb_count[(int64_t)g * NB1 + b] = cnt[b];
+            s += cnt[b];
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+        const int b = bp.meta[r0 + r].bin;
+        const int k = atomicAdd(&pos[b < 0 ? bp.NB : b], 1);
+        bp.read_perm[r0 + k] = r0 + r;
+    }
+}
+
+__device__ __forceinline__ int key_of(int bin, int hl)
+{
+    return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
+}
+
+// One block per group: add its alignment counts to the (bin, hap length)
+// histogram.
+__global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
+{
+    const int g = blockIdx.x;
+    const int NB1 = bp.NB + 1;
+    const int h0 = bp.hprefix[g], nh = bp.nhaps[g];
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        const int hl = bp.hap_len[h0 + h];
+        if (hl <= 0)
+            continue;
+        for (int b = 0; b < bp.NB; ++b) {
+            const int n = bp.gb_count[(int64_t)g * NB1 + b];
+            if (n > 0)
+                atomicAdd(&bp.key_count[key_of(b, hl)], n);
+        }
+    }
+}
+
+// Exclusive scan of the key histogram (one block); per-bin ranges.
+__global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    const int n = bp.NB * kHapBuckets;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < n ? bp.key_count[i] : 0;
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o)
+                x += y;
+        }
+        if (lane == 63)
+            wsum[wv] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int k = 0; k < 16; ++k) {
+                const int t = wsum[k];
+                wsum[k] = s;
+                s += t;
+            }
+        }
+        __syncthreads();
+        const int incl = x + wsum[wv] + carry;
+        if (i < n) {
+            bp.key_cursor[i] = incl - v;
+            if (i % kHapBuckets == 0)
+                bp.bin_range[2 * (i / kHapBuckets)] = incl - v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023)
+            carry = incl;
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < bp.NB; b += blockDim.x) {
+        const int end = (b + 1 < bp.NB) ? bp.bin_range[2 * (b + 1)] : carry;
+        bp.bin_range[2 * b + 1] = end - bp.bin_range[2 * b];
+    }
+}
+
+// One block per group: reserve each (hap, bin) run and write its items;
+// alignments with an empty read or hap get -inf directly (align_host returns
+// log10(0) - log10(c) for them).
+__global__ __launch_bounds__(256) void k_scatter(BinParams bp)
+{
+    const int g = blockIdx.x;
+    const int NB1 = bp.NB + 1;
+    const int r0 = bp.rprefix[g], nr = bp.nreads[g];
+    const int h0 = bp.hprefix[g], nh = bp.nhaps[g];
+    const int64_t A = bp.align_off[g];
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        const int hl = bp.hap_len[h0 + h];
+        if (hl <= 0) {
+            for (int r = 0; r < nr; ++r)
+                bp.scores[A + (int64_t)r * nh + h] = -INFINITY;
+            continue;
+        }
+        for (int b = 0; b < NB1; ++b) {
+            const int n = bp.gb_count[(int64_t)g * NB1 + b];
+            if (n <= 0)
+                continue;
+            const int st = bp.gb_start[(int64_t)g * NB1 + b];
+            if (b == bp.NB) {
+                for (int k = 0; k < n; ++k) {
+                    const int rd = bp.read_perm[r0 + st + k];
+                    bp.scores[A + (int64_t)(rd - r0) * nh + h] = -INFINITY;
+                }
+                continue;
+            }
+            const int base = atomicAdd(&bp.key_cursor[key_of(b, hl)], n);
+            for (int k = 0; k < n; ++k) {
+                const int rd = bp.read_perm[r0 + st + k];
+                WorkItem it;
+                it.read = rd;
+                it.hap = h0 + h;
+                it.out = (int32_t)(A + (int64_t)(rd - r0) * nh + h);
+                it.pad = 0;
+                bp.items[base + k] = it;
+            }
+        }
+    }
+}
+
+__global__ void k_add_count(int64_t* total, const int32_t* count)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *total += *count;
+}
+
+hipError_t launch_add_count(int64_t* total, const int32_t* count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, st, total, count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+hipError_t run_binning(const BinParams& bp, hipStream_t st)
+{
+    const int tpb = 256;
+    auto blocks_for = [](int64_t n, int tpb_) {
+        int64_t b = (n + tpb_ - 1) / tpb_;
+        return (int)(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+    };
+    const int NB1 = bp.NB + 1;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for((int64_t)bp.NB * kHapBuckets, tpb)), dim3(tpb), 0,
+                       st, bp.key_count, (int64_t)bp.NB * kHapBuckets);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.hmax, (int64_t)1);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.rescue_count, (int64_t)1);
+    if (bp.read_bytes > 0)
+        hipLaunchKernelGGL(k_encode_reads, dim3(blocks_for(bp.read_bytes, tpb)), dim3(tpb), 0, st, bp);
+    if (bp.hap_bytes > 0)
+        hipLaunchKernelGGL(k_encode_haps, dim3(blocks_for(bp.hap_bytes, tpb)), dim3(tpb), 0, st, bp);
+    if (bp.nH > 0)
+        hipLaunchKernelGGL(k_hap_max, dim3(blocks_for(bp.nH, tpb)), dim3(tpb), 0, st, bp);
+    if (bp.nR > 0)
+        hipLaunchKernelGGL(k_read_meta, dim3((bp.nR + tpb - 1) / tpb), dim3(tpb), 0, st, bp);
+    hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
+    if (bp.G > 0) {
+        hipLaunchKernelGGL(k_group_bins, dim3(bp.G), dim3(256), 0, st, bp);
+        hipLaunchKernelGGL(k_count_keys, dim3(bp.G), dim3(256), 0, st, bp);
+    }
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, bp);
+    if (bp.G > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(bp.G), dim3(256), 0, st, bp);
+    (void)NB1;
+    return hipGetLastError();
+}
+
+}  // namespace phmm

@@ -1,0 +1,883 @@
+// The engine: device memory, work-list construction, kernel dispatch, FP64
+// rescue, the chunked host pipeline, and the C ABI around them.
+//
+// Mirrors GpuPairHMMWorker (gpuworker.cuh:381-973): the constructor uploads
+// the phred table and creates streams; processBatch... becomes
+// Engine::score_batch (chunking by transfer_batchsize at record granularity,
+// copies overlapped with compute) on top of Engine::score_device (the device
+// part: binning, per-bin forward kernels, rescue).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "engine.hpp"
+
+namespace phmm {
+
+// Default tile table: tiles (lanes x rows) ascending.  A read of length L
+// uses the smallest tile >= L + 1.
+const phmm_tile_config kDefaultTiles[16] = {
+    {2, 8},  {4, 8},  {6, 8},  {4, 16}, {5, 16}, {6, 16},  {7, 16},  {8, 16},
+    {8, 20}, {8, 24}, {8, 28}, {8, 32}, {10, 32}, {12, 32}, {16, 32}, {32, 32},
+};
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+const char* last_error() { return g_last_error.c_str(); }
+
+// Reads and clears the device error flag the multi-tile and rescue kernels
+// raise when the spill arena cannot hold one alignment.
+int Engine::take_error_flag()
+{
+    if (!smallints.p)
+        return PHMM_OK;
+    int32_t* d_flag = static_cast<int32_t*>(smallints.p) + 2;
+    int32_t flag = 0;
+    int rc;
+    if ((rc = check(hipMemcpy(&flag, d_flag, sizeof flag, hipMemcpyDeviceToHost), "error flag")))
+        return rc;
+    if (!flag)
+        return PHMM_OK;
+    const int32_t zero = 0;
+    hipMemcpy(d_flag, &zero, sizeof zero, hipMemcpyHostToDevice);
+    return set_error(PHMM_ERANGE, "multi-tile/rescue spill arena too small for a batch (flag " +
+                                      std::to_string(flag) + ")");
+}
+
+Engine::~Engine()
+{
+    hipSetDevice(device);
+    if (stream)
+        hipStreamSynchronize(stream);
+    if (copy_stream)
+        hipStreamSynchronize(copy_stream);
+    for (DevBuf* b : {&rows, &hapc, &meta, &gb_count, &gb_start, &read_perm, &align_off, &key_count,
+                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &spill})
+        if (b->p)
+            hipFree(b->p);
+    for (auto& s : slots)
+        free_slot(s);
+    if (d_rescued_total)
+        hipFree(d_rescued_total);
+    if (h_bin_range)
+        hipHostFree(h_bin_range);
+    if (h_small)
+        hipHostFree(h_small);
+    if (d_ph2pr)
+        hipFree(d_ph2pr);
+    for (auto* v : {&ev_free, &ev_pending})
+        for (auto& pr : *v) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+    if (stream)
+        hipStreamDestroy(stream);
+    if (copy_stream)
+        hipStreamDestroy(copy_stream);
+}
+
+int Engine::ensure(DevBuf& b, size_t bytes, const char* what)
+{
+    if (bytes <= b.cap && b.p)
+        return PHMM_OK;
+    if (b.p) {
+        hipDeviceSynchronize();  // in-flight kernels may still use it
+        hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t nb = std::max<size_t>(bytes, 256);
+    nb = (nb + 255) & ~(size_t)255;
+    if (hipMalloc(&b.p, nb) != hipSuccess) {
+        hipGetLastError();
+        b.p = nullptr;
+        return set_error(PHMM_ENOMEM, std::string("device allocation failed: ") + what + " (" +
+                                          std::to_string(nb) + " bytes)");
+    }
+    b.cap = nb;
+    return PHMM_OK;
+}
+
+int Engine::ensure_zeroed(DevBuf& b, size_t bytes, const char* what)
+{
+    if (bytes <= b.cap && b.p)
+        return PHMM_OK;
+    int rc = ensure(b, bytes, what);
+    if (rc)
+        return rc;
+    return check(hipMemset(b.p, 0, b.cap), "memset");
+}
+
+int Engine::ensure_pin(PinBuf& b, size_t bytes, const char* what)
+{
+    if (bytes <= b.cap && b.p)
+        return PHMM_OK;
+    if (b.p) {
+        hipHostFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t nb = std::max<size_t>(bytes + bytes / 4, 4096);
+    if (hipHostMalloc(&b.p, nb, hipHostMallocDefault) != hipSuccess) {
+        hipGetLastError();
+        b.p = nullptr;
+        return set_error(PHMM_ENOMEM, std::string("pinned allocation failed: ") + what);
+    }
+    b.cap = nb;
+    return PHMM_OK;
+}
+
+// Adds the elapsed time of every pending event pair (waits for them).
+int Engine::finish_timing()
+{
+    for (auto& pr : ev_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(pr.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
+            stats.kernel_seconds += ms * 1e-3;
+        else
+            hipGetLastError();
+        ev_free.push_back(pr);
+    }
+    ev_pending.clear();
+    return PHMM_OK;
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The device part of a batch.  `v` holds device pointers (counts on the
+// host) and must be self-contained (reads_prefix[0] == haps_prefix[0] == 0).
+int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st)
+{
+    if (v.num_groups < 0 || v.num_reads < 0 || v.num_haps < 0 || v.num_alignments < 0)
+        return set_error(PHMM_EINVAL, "score_device: negative counts");
+    if (v.num_alignments > INT32_MAX)
+        return set_error(PHMM_ERANGE, "score_device: more than 2^31-1 alignments in one batch");
+    if (v.num_alignments == 0 || v.num_groups == 0)
+        return PHMM_OK;
+    if (!d_scores || !v.read_lengths || !v.hap_lengths || !v.read_offsets || !v.hap_offsets ||
+        !v.num_reads_per_group || !v.num_haps_per_group || !v.reads_prefix || !v.haps_prefix ||
+        (v.num_reads > 0 && (!v.read_data || !v.base_quals || !v.ins_quals || !v.del_quals ||
+                             !v.gcp_quals)) ||
+        (v.num_haps > 0 && !v.hap_data))
+        return set_error(PHMM_EINVAL, "score_device: null device pointer in batch view");
+    const int ntiles = (int)tiles.size();
+    const int NB = 2 * ntiles + 1;
+    const int G = v.num_groups;
+    const int64_t nA = v.num_alignments;
+    int rc;
+    if ((rc = ensure(rows, sizeof(uint32_t) * (size_t)std::max<int64_t>(v.read_bytes, 1), "rows")) ||
+        (rc = ensure(hapc, (size_t)std::max<int64_t>(v.hap_bytes, 4) + 16, "hap codes")) ||
+        (rc = ensure(meta, sizeof(ReadMeta) * (size_t)std::max(v.num_reads, 1), "read meta")) ||
+        (rc = ensure(gb_count, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
+        (rc = ensure(gb_start, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
+        (rc = ensure(read_perm, sizeof(int32_t) * (size_t)std::max(v.num_reads, 1), "read perm")) ||
+        (rc = ensure(align_off, sizeof(int64_t) * (size_t)(G + 1), "align offsets")) ||
+        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
+        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
+        (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NB, "bin ranges")) ||
+        (rc = ensure_zeroed(smallints, 256, "counters")) ||
+        (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
+        (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")))
+        return rc;
+    const size_t spill_bytes = (size_t)256 << 20;
+    if ((rc = ensure(spill, spill_bytes, "spill")))
+        return rc;
+
+    int32_t* sm = static_cast<int32_t*>(smallints.p);
+    int32_t* d_hmax = sm + 0;
+    int32_t* d_rescue_count = sm + 1;
+    int32_t* d_error = sm + 2;
+
+    BinParams bp{};
+    bp.read_data = v.read_data;
+    bp.hap_data = v.hap_data;
+    bp.bq = v.base_quals;
+    bp.iq = v.ins_quals;
+    bp.dq = v.del_quals;
+    bp.gcp = v.gcp_quals;
+    bp.read_len = v.read_lengths;
+    bp.hap_len = v.hap_lengths;
+    bp.read_off = v.read_offsets;
+    bp.hap_off = v.hap_offsets;
+    bp.nreads = v.num_reads_per_group;
+    bp.nhaps = v.num_haps_per_group;
+    bp.rprefix = v.reads_prefix;
+    bp.hprefix = v.haps_prefix;
+    bp.G = G;
+    bp.nR = v.num_reads;
+    bp.nH = v.num_haps;
+    bp.NB = NB;
+    bp.read_bytes = v.read_bytes;
+    bp.hap_bytes = v.hap_bytes;
+    bp.rows = static_cast<uint32_t*>(rows.p);
+    bp.hapc = static_cast<uint8_t*>(hapc.p);
+    bp.meta = static_cast<ReadMeta*>(meta.p);
+    bp.gb_count = static_cast<int32_t*>(gb_count.p);
+    bp.gb_start = static_cast<int32_t*>(gb_start.p);
+    bp.read_perm = static_cast<int32_t*>(read_perm.p);
+    bp.align_off = static_cast<int64_t*>(align_off.p);
+    bp.key_count = static_cast<int32_t*>(key_count.p);
+    bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
+    bp.bin_range = static_cast<int32_t*>(bin_range.p);
+    bp.hmax = d_hmax;
+    bp.rescue_count = d_rescue_count;
+    bp.error_flag = d_error;
+    bp.items = static_cast<WorkItem*>(items.p);
+    bp.scores = d_scores;
+    bp.ph2pr = d_ph2pr;
+    bp.ntiles = ntiles;
+    for (int i = 0; i < ntiles; ++i)
+        bp.tiles[i] = tiles[(size_t)i].lanes * tiles[(size_t)i].rows;
+    bp.semantics = opt.semantics;
+    bp.allow_fast = 1;
+    if ((rc = check(run_binning(bp, st), "binning kernels")))
+        return rc;
+
+    FwdParams fp{};
+    fp.items = bp.items;
+    fp.rows = bp.rows;
+    fp.hapc = bp.hapc;
+    fp.gcp = v.gcp_quals;
+    fp.read_len = v.read_lengths;
+    fp.hap_len = v.hap_lengths;
+    fp.read_off = v.read_offsets;
+    fp.hap_off = v.hap_offsets;
+    fp.meta = bp.meta;
+    fp.ph2pr = d_ph2pr;
+    fp.scores = d_scores;
+    fp.rescue_list = static_cast<int32_t*>(rescue_list.p);
+    fp.rescue_count = d_rescue_count;
+    fp.rescue_min = opt.rescue_min_sum;
+    fp.rescue_on = opt.fp64_rescue;
+    fp.semantics = opt.semantics;
+
+    const bool timed = opt.time_kernels != 0;
+    std::pair<hipEvent_t, hipEvent_t> evp{nullptr, nullptr};
+    if (timed) {
+        if (ev_free.empty()) {
+            if ((rc = check(hipEventCreate(&evp.first), "event")) ||
+                (rc = check(hipEventCreate(&evp.second), "event")))
+                return rc;
+        } else {
+            evp = ev_free.back();
+            ev_free.pop_back();
+        }
+    }
+    const int persist_blocks = num_cus * 8;
+    if (opt.mode == PHMM_MODE_PLAIN) {
+        // one host round trip per batch for the per-bin sizes (the reference
+        // syncs per chunk for its per-partition counts, gpuworker.cuh:852-854)
+        if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NB,
+                                       hipMemcpyDeviceToHost, st),
+                        "bin range readback")) ||
+            (rc = check(hipStreamSynchronize(st), "bin range sync")))
+            return rc;
+        if (timed)
+            hipEventRecord(evp.first, st);
+        for (int b = 0; b < 2 * ntiles; ++b) {
+            const int begin = h_bin_range[2 * b], count = h_bin_range[2 * b + 1];
+            if (count <= 0)
+                continue;
+            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
+            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, nullptr, begin, count,
+                                            tc.lanes, 0, st),
+                            "forward kernel")))
+                return rc;
+            if (b % 2 == 0)
+                stats.fast_alignments += count;
+        }
+        const int mb = 2 * ntiles;
+        if (h_bin_range[2 * mb + 1] > 0) {
+            stats.multitile_alignments += h_bin_range[2 * mb + 1];
+            if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
+                                                bp.bin_range + 2 * mb + 1, d_hmax,
+                                                static_cast<float*>(spill.p),
+                                                (int64_t)(spill.cap / sizeof(float)), d_error,
+                                                std::min(persist_blocks, h_bin_range[2 * mb + 1]),
+                                                st),
+                            "multi-tile kernel")))
+                return rc;
+        }
+    } else {
+        if (timed)
+            hipEventRecord(evp.first, st);
+        for (int b = 0; b < 2 * ntiles; ++b) {
+            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
+            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, bp.bin_range + 2 * b, 0, 0,
+                                            tc.lanes, persist_blocks, st),
+                            "forward kernel")))
+                return rc;
+        }
+        const int mb = 2 * ntiles;
+        if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
+                                            bp.bin_range + 2 * mb + 1, d_hmax,
+                                            static_cast<float*>(spill.p),
+                                            (int64_t)(spill.cap / sizeof(float)), d_error,
+                                            persist_blocks, st),
+                        "multi-tile kernel")))
+            return rc;
+    }
+    if (timed) {
+        hipEventRecord(evp.second, st);
+        ev_pending.push_back(evp);
+    }
+    if (opt.fp64_rescue) {
+        if ((rc = check(launch_fwd_wave_f64(fp, fp.rescue_list, nullptr, d_rescue_count, d_hmax,
+                                            static_cast<double*>(spill.p),
+                                            (int64_t)(spill.cap / sizeof(double)), d_error,
+                                            persist_blocks, st),
+                        "FP64 rescue kernel")) ||
+            (rc = check(launch_add_count(d_rescued_total, d_rescue_count, st), "rescue count")))
+            return rc;
+    }
+    stats.alignments += nA;
+    return PHMM_OK;
+}
+
+void Engine::free_slot(Slot& s)
+{
+    if (s.dev.p)
+        hipFree(s.dev.p);
+    if (s.scores.p)
+        hipFree(s.scores.p);
+    if (s.pin.p)
+        hipHostFree(s.pin.p);
+    if (s.pin_scores.p)
+        hipHostFree(s.pin_scores.p);
+    if (s.h2d_done)
+        hipEventDestroy(s.h2d_done);
+    if (s.compute_done)
+        hipEventDestroy(s.compute_done);
+    s = Slot{};
+}
+
+int Engine::launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1)
+{
+    struct Layout {
+        size_t read_data, hap_data, bq, iq, dq, gq, rl, hl, ro, ho, nr, nh, rp, hp, total;
+    };
+    const int G = g1 - g0;
+    const int r0 = v.reads_prefix[g0], r1 = v.reads_prefix[g1];
+    const int h0 = v.haps_prefix[g0], h1 = v.haps_prefix[g1];
+    const int nR = r1 - r0, nH = h1 - h0;
+    const int64_t rb0 = nR > 0 ? v.read_offsets[r0] : 0;
+    const int64_t rb1 = nR > 0 ? v.read_offsets[r1 - 1] + padded_length(v.read_lengths[r1 - 1]) : 0;
+    const int64_t hb0 = nH > 0 ? v.hap_offsets[h0] : 0;
+    const int64_t hb1 = nH > 0 ? v.hap_offsets[h1 - 1] + padded_length(v.hap_lengths[h1 - 1]) : 0;
+    const int64_t rbytes = rb1 - rb0, hbytes = hb1 - hb0;
+    int64_t nA = 0;
+    for (int g = g0; g < g1; ++g)
+        nA += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+    Layout L{};
+    {
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = o;
+            o = align256(o + bytes);
+            return at;
+        };
+        L.read_data = take((size_t)rbytes);
+        L.bq = take((size_t)rbytes);
+        L.iq = take((size_t)rbytes);
+        L.dq = take((size_t)rbytes);
+        L.gq = take((size_t)rbytes);
+        L.hap_data = take((size_t)hbytes);
+        L.rl = take(4 * (size_t)nR);
+        L.hl = take(4 * (size_t)nH);
+        L.ro = take(4 * (size_t)(nR + 1));
+        L.ho = take(4 * (size_t)(nH + 1));
+        L.nr = take(4 * (size_t)G);
+        L.nh = take(4 * (size_t)G);
+        L.rp = take(4 * (size_t)(G + 1));
+        L.hp = take(4 * (size_t)(G + 1));
+        L.total = o;
+    }
+    int rc;
+    if ((rc = ensure_pin(s.pin, L.total, "staging")) || (rc = ensure(s.dev, L.total, "chunk")) ||
+        (rc = ensure(s.scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1), "scores")) ||
+        (rc = ensure_pin(s.pin_scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1),
+                         "score staging")))
+        return rc;
+    // a self-contained, rebased copy of the chunk
+    char* P = static_cast<char*>(s.pin.p);
+    std::memcpy(P + L.read_data, v.read_data + rb0, (size_t)rbytes);
+    std::memcpy(P + L.bq, v.base_quals + rb0, (size_t)rbytes);
+    std::memcpy(P + L.iq, v.ins_quals + rb0, (size_t)rbytes);
+    std::memcpy(P + L.dq, v.del_quals + rb0, (size_t)rbytes);
+    std::memcpy(P + L.gq, v.gcp_quals + rb0, (size_t)rbytes);
+    std::memcpy(P + L.hap_data, v.hap_data + hb0, (size_t)hbytes);
+    std::memcpy(P + L.rl, v.read_lengths + r0, 4 * (size_t)nR);
+    std::memcpy(P + L.hl, v.hap_lengths + h0, 4 * (size_t)nH);
+    int32_t* ro = reinterpret_cast<int32_t*>(P + L.ro);
+    for (int i = 0; i <= nR; ++i)
+        ro[i] = (int32_t)((i < nR ? v.read_offsets[r0 + i] : rb1) - rb0);
+    int32_t* ho = reinterpret_cast<int32_t*>(P + L.ho);
+    for (int i = 0; i <= nH; ++i)
+        ho[i] = (int32_t)((i < nH ? v.hap_offsets[h0 + i] : hb1) - hb0);
+    std::memcpy(P + L.nr, v.num_reads_per_group + g0, 4 * (size_t)G);
+    std::memcpy(P + L.nh, v.num_haps_per_group + g0, 4 * (size_t)G);
+    int32_t* rp = reinterpret_cast<int32_t*>(P + L.rp);
+    int32_t* hp = reinterpret_cast<int32_t*>(P + L.hp);
+    for (int g = 0; g <= G; ++g) {
+        rp[g] = v.reads_prefix[g0 + g] - r0;
+        hp[g] = v.haps_prefix[g0 + g] - h0;
+    }
+    if (!s.h2d_done) {
+        if ((rc = check(hipEventCreateWithFlags(&s.h2d_done, hipEventDisableTiming), "event")) ||
+            (rc = check(hipEventCreateWithFlags(&s.compute_done, hipEventDisableTiming), "event")))
+            return rc;
+    }
+    if ((rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.total, hipMemcpyHostToDevice, copy_stream),
+                    "H2D copy")) ||
+        (rc = check(hipEventRecord(s.h2d_done, copy_stream), "event")) ||
+        (rc = check(hipStreamWaitEvent(stream, s.h2d_done, 0), "event wait")))
+        return rc;
+    char* D = static_cast<char*>(s.dev.p);
+    phmm_batch_view dv{};
+    dv.num_groups = G;
+    dv.num_reads = nR;
+    dv.num_haps = nH;
+    dv.read_bytes = rbytes;
+    dv.hap_bytes = hbytes;
+    dv.num_alignments = nA;
+    dv.read_data = reinterpret_cast<const uint8_t*>(D + L.read_data);
+    dv.hap_data = reinterpret_cast<const uint8_t*>(D + L.hap_data);
+    dv.base_quals = reinterpret_cast<const uint8_t*>(D + L.bq);
+    dv.ins_quals = reinterpret_cast<const uint8_t*>(D + L.iq);
+    dv.del_quals = reinterpret_cast<const uint8_t*>(D + L.dq);
+    dv.gcp_quals = reinterpret_cast<const uint8_t*>(D + L.gq);
+    dv.read_lengths = reinterpret_cast<const int32_t*>(D + L.rl);
+    dv.hap_lengths = reinterpret_cast<const int32_t*>(D + L.hl);
+    dv.read_offsets = reinterpret_cast<const int32_t*>(D + L.ro);
+    dv.hap_offsets = reinterpret_cast<const int32_t*>(D + L.ho);
+    dv.num_reads_per_group = reinterpret_cast<const int32_t*>(D + L.nr);
+    dv.num_haps_per_group = reinterpret_cast<const int32_t*>(D + L.nh);
+    dv.reads_prefix = reinterpret_cast<const int32_t*>(D + L.rp);
+    dv.haps_prefix = reinterpret_cast<const int32_t*>(D + L.hp);
+    if ((rc = score_device(dv, static_cast<float*>(s.scores.p), stream)))
+        return rc;
+    if ((rc = check(hipMemcpyAsync(s.pin_scores.p, s.scores.p, sizeof(float) * (size_t)nA,
+                                   hipMemcpyDeviceToHost, stream),
+                    "D2H copy")) ||
+        (rc = check(hipEventRecord(s.compute_done, stream), "event")))
+        return rc;
+    s.busy = true;
+    s.n_scores = nA;
+    return PHMM_OK;
+}
+
+// Host batch in, host scores out.  Chunks close at the first record where
+// the running alignment count reaches transfer_batchsize
+// (gpuworker.cuh:659-669); chunk i+1 is staged and copied while chunk i
+// computes.  A slot is restaged only after its previous chunk completed, so
+// no copy can overtake the compute that reads the slot.
+int Engine::score_batch(const phmm_batch_view& v, float* out)
+{
+    const std::string verr = validate_host_view(v);
+    if (!verr.empty())
+        return set_error(PHMM_EINVAL, "phmm_score_batch: " + verr);
+    if (v.num_alignments == 0)
+        return PHMM_OK;
+    if (!out)
+        return set_error(PHMM_EINVAL, "phmm_score_batch: null output");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t tb = opt.transfer_batchsize > 0 ? opt.transfer_batchsize : INT64_MAX;
+    std::vector<std::pair<int, int>> chunks;
+    {
+        int g0 = 0;
+        int64_t acc = 0;
+        for (int g = 0; g < v.num_groups; ++g) {
+            acc += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+            if (acc >= tb) {
+                chunks.push_back({g0, g + 1});
+                g0 = g + 1;
+                acc = 0;
+            }
+        }
+        if (g0 < v.num_groups)
+            chunks.push_back({g0, v.num_groups});
+    }
+    int rc;
+    if ((rc = take_error_flag()))
+        return rc;
+    int64_t out_off[2] = {0, 0};
+    auto drain = [&](int si) -> int {
+        Slot& s = slots[si];
+        if (!s.busy)
+            return PHMM_OK;
+        int r2;
+        if ((r2 = check(hipEventSynchronize(s.compute_done), "chunk sync")))
+            return r2;
+        std::memcpy(out + out_off[si], s.pin_scores.p, sizeof(float) * (size_t)s.n_scores);
+        s.busy = false;
+        return PHMM_OK;
+    };
+    int64_t next_off = 0;
+    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+        const int si = (int)(ci % 2);
+        if ((rc = drain(si)))
+            return rc;
+        if ((rc = launch_chunk(slots[si], v, chunks[ci].first, chunks[ci].second)))
+            return rc;
+        out_off[si] = next_off;
+        next_off += slots[si].n_scores;
+    }
+    for (int si = 0; si < 2; ++si)
+        if ((rc = drain(si)))
+            return rc;
+    if ((rc = check(hipStreamSynchronize(stream), "final sync")))
+        return rc;
+    finish_timing();
+    if ((rc = take_error_flag()))
+        return rc;
+    stats.cells += phmm_batch_cells(&v);
+    stats.engine_seconds +=
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PHMM_OK;
+}
+
+}  // namespace phmm
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using phmm::Engine;
+
+struct phmm_engine {
+    Engine e;
+    std::mutex mu;
+};
+
+struct phmm_device_batch {
+    phmm_engine* eng = nullptr;
+    void* mem = nullptr;
+    phmm_batch_view view{};
+};
+
+namespace phmm {
+Engine* engine_of(phmm_engine* h, std::mutex** mu)
+{
+    if (mu)
+        *mu = &h->mu;
+    return &h->e;
+}
+}  // namespace phmm
+
+extern "C" int phmm_abi_version(void) { return PHMM_ABI_VERSION; }
+extern "C" const char* phmm_last_error(void) { return phmm::last_error(); }
+
+extern "C" void phmm_options_default(phmm_options* o)
+{
+    if (!o)
+        return;
+    std::memset(o, 0, sizeof *o);
+    o->transfer_batchsize = 1000000;  // common.cuh:19
+    o->mode = PHMM_MODE_BATCHED;
+    o->semantics = PHMM_SEM_GPU;
+    o->fp64_rescue = 1;
+    o->rescue_min_sum = 1e-28f;
+    o->verbose = 0;
+}
+
+extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engine** out)
+{
+    if (!out)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: null output");
+    *out = nullptr;
+    phmm_options o;
+    if (opt)
+        o = *opt;
+    else
+        phmm_options_default(&o);
+    if (o.mode != PHMM_MODE_PLAIN && o.mode != PHMM_MODE_BATCHED)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown mode");
+    if (o.semantics != PHMM_SEM_GPU && o.semantics != PHMM_SEM_CPU)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown semantics");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        hipGetLastError();
+        return phmm::fail(PHMM_EDEVICE, "phmm_engine_create: no HIP device");
+    }
+    if (device < 0 || device >= ndev)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: device out of range");
+    auto* h = new (std::nothrow) phmm_engine;
+    if (!h)
+        return phmm::fail(PHMM_ENOMEM, "phmm_engine_create: out of memory");
+    Engine& e = h->e;
+    e.device = device;
+    e.opt = o;
+    e.tiles.assign(std::begin(phmm::kDefaultTiles), std::end(phmm::kDefaultTiles));
+    int rc;
+    auto bail = [&](int code) {
+        const std::string m = e.err.empty() ? phmm::last_error() : e.err;
+        delete h;
+        return phmm::fail(code, m);
+    };
+    if ((rc = e.check(hipSetDevice(device), "hipSetDevice")))
+        return bail(rc);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+        e.num_cus = prop.multiProcessorCount;
+    else
+        hipGetLastError();
+    if ((rc = e.check(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking), "stream")) ||
+        (rc = e.check(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking), "stream")) ||
+        (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
+        (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipHostMalloc(&e.h_bin_range, 2 * sizeof(int32_t) * (2 * phmm::kMaxTiles + 1),
+                                    hipHostMallocDefault),
+                      "pinned")) ||
+        (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")))
+        return bail(rc);
+    // ph2pr[q] = (float)10^(-q/10) (gpuworker.cuh:473-486)
+    float table[128];
+    for (int q = 0; q < 128; ++q)
+        table[q] = (float)std::pow(10.0, (double)(-q) / 10.0);
+    if ((rc = e.check(hipMemcpy(e.d_ph2pr, table, sizeof table, hipMemcpyHostToDevice), "ph2pr")))
+        return bail(rc);
+    *out = h;
+    return PHMM_OK;
+}
+
+extern "C" void phmm_engine_destroy(phmm_engine* h) { delete h; }
+
+extern "C" const char* phmm_engine_last_error(const phmm_engine* h)
+{
+    return h ? h->e.err.c_str() : phmm::last_error();
+}
+
+extern "C" int phmm_engine_stats(const phmm_engine* h, phmm_stats* out)
+{
+    if (!h || !out)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_stats: null argument");
+    auto* m = const_cast<phmm_engine*>(h);
+    std::lock_guard<std::mutex> lk(m->mu);
+    hipSetDevice(m->e.device);
+    hipStreamSynchronize(m->e.stream);
+    m->e.finish_timing();
+    int64_t rescued = 0;
+    if (m->e.d_rescued_total) {
+        if (hipMemcpy(&rescued, m->e.d_rescued_total, sizeof rescued, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            hipGetLastError();
+    }
+    *out = h->e.stats;
+    out->rescued = rescued;
+    return PHMM_OK;
+}
+
+extern "C" int phmm_engine_reset_stats(phmm_engine* h)
+{
+    if (!h)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_reset_stats: null engine");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->e.finish_timing();
+    h->e.stats = phmm_stats{};
+    if (h->e.d_rescued_total) {
+        hipSetDevice(h->e.device);
+        hipDeviceSynchronize();
+        if (hipMemset(h->e.d_rescued_total, 0, sizeof(int64_t)) != hipSuccess)
+            hipGetLastError();
+    }
+    return PHMM_OK;
+}
+
+extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, int n)
+{
+    if (!h || !t || n <= 0 || n > phmm::kMaxTiles)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_set_tiles: bad arguments");
+    int prev = 0;
+    for (int i = 0; i < n; ++i) {
+        const int tile = t[i].lanes * t[i].rows;
+        if (t[i].lanes < 1 || t[i].lanes > 64 || t[i].rows < 4 || t[i].rows > 32 ||
+            t[i].rows % 4 != 0 || tile <= prev)
+            return phmm::fail(PHMM_EINVAL,
+                              "phmm_engine_set_tiles: lanes 1..64, rows 4..32 step 4, tiles ascending");
+        prev = tile;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->e.tiles.assign(t, t + n);
+    return PHMM_OK;
+}
+
+extern "C" int phmm_engine_get_tiles(const phmm_engine* h, phmm_tile_config* t, int cap, int* n)
+{
+    if (!h || !n)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_get_tiles: null argument");
+    *n = (int)h->e.tiles.size();
+    for (int i = 0; i < *n && i < cap && t; ++i)
+        t[i] = h->e.tiles[(size_t)i];
+    return PHMM_OK;
+}
+
+extern "C" int phmm_score_batch(phmm_engine* h, const phmm_batch_view* b, float* out)
+{
+    if (!h || !b)
+        return phmm::fail(PHMM_EINVAL, "phmm_score_batch: null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipSetDevice(h->e.device);
+    return h->e.score_batch(*b, out);
+}
+
+extern "C" int phmm_score_device(phmm_engine* h, const phmm_batch_view* b, float* d_scores,
+                                 void* hip_stream)
+{
+    if (!h || !b)
+        return phmm::fail(PHMM_EINVAL, "phmm_score_device: null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipSetDevice(h->e.device);
+    hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : h->e.stream;
+    return h->e.score_device(*b, d_scores, st);
+}
+
+extern "C" int phmm_engine_synchronize(phmm_engine* h)
+{
+    if (!h)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_synchronize: null engine");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipSetDevice(h->e.device);
+    int rc = h->e.check(hipDeviceSynchronize(), "synchronize");
+    if (rc)
+        return rc;
+    h->e.finish_timing();
+    return h->e.take_error_flag();
+}
+
+extern "C" int phmm_device_batch_upload(phmm_engine* h, const phmm_batch_view* v,
+                                        phmm_device_batch** out)
+{
+    if (!h || !v || !out)
+        return phmm::fail(PHMM_EINVAL, "phmm_device_batch_upload: null argument");
+    *out = nullptr;
+    const std::string verr = phmm::validate_host_view(*v);
+    if (!verr.empty())
+        return phmm::fail(PHMM_EINVAL, "phmm_device_batch_upload: " + verr);
+    std::lock_guard<std::mutex> lk(h->mu);
+    Engine& e = h->e;
+    hipSetDevice(e.device);
+    const int G = v->num_groups, nR = v->num_reads, nH = v->num_haps;
+    size_t off[14];
+    size_t o = 0;
+    const size_t sizes[14] = {(size_t)v->read_bytes, (size_t)v->read_bytes, (size_t)v->read_bytes,
+                              (size_t)v->read_bytes, (size_t)v->read_bytes, (size_t)v->hap_bytes,
+                              4 * (size_t)nR,          4 * (size_t)nH,          4 * (size_t)(nR + 1),
+                              4 * (size_t)(nH + 1),    4 * (size_t)G,           4 * (size_t)G,
+                              4 * (size_t)(G + 1),     4 * (size_t)(G + 1)};
+    const void* src[14] = {v->read_data,   v->base_quals,   v->ins_quals,           v->del_quals,
+                           v->gcp_quals,   v->hap_data,     v->read_lengths,        v->hap_lengths,
+                           v->read_offsets, v->hap_offsets, v->num_reads_per_group, v->num_haps_per_group,
+                           v->reads_prefix, v->haps_prefix};
+    for (int i = 0; i < 14; ++i) {
+        off[i] = o;
+        o = phmm::align256(o + sizes[i]);
+    }
+    auto* db = new (std::nothrow) phmm_device_batch;
+    if (!db)
+        return phmm::fail(PHMM_ENOMEM, "phmm_device_batch_upload: out of memory");
+    if (hipMalloc(&db->mem, std::max<size_t>(o, 256)) != hipSuccess) {
+        hipGetLastError();
+        delete db;
+        return e.set_error(PHMM_ENOMEM, "phmm_device_batch_upload: device allocation failed");
+    }
+    char* D = static_cast<char*>(db->mem);
+    for (int i = 0; i < 14; ++i)
+        if (sizes[i] && src[i] &&
+            hipMemcpy(D + off[i], src[i], sizes[i], hipMemcpyHostToDevice) != hipSuccess) {
+            hipGetLastError();
+            hipFree(db->mem);
+            delete db;
+            return e.set_error(PHMM_EDEVICE, "phmm_device_batch_upload: copy failed");
+        }
+    phmm_batch_view dv = *v;
+    dv.read_data = reinterpret_cast<const uint8_t*>(D + off[0]);
+    dv.base_quals = reinterpret_cast<const uint8_t*>(D + off[1]);
+    dv.ins_quals = reinterpret_cast<const uint8_t*>(D + off[2]);
+    dv.del_quals = reinterpret_cast<const uint8_t*>(D + off[3]);
+    dv.gcp_quals = reinterpret_cast<const uint8_t*>(D + off[4]);
+    dv.hap_data = reinterpret_cast<const uint8_t*>(D + off[5]);
+    dv.read_lengths = reinterpret_cast<const int32_t*>(D + off[6]);
+    dv.hap_lengths = reinterpret_cast<const int32_t*>(D + off[7]);
+    dv.read_offsets = reinterpret_cast<const int32_t*>(D + off[8]);
+    dv.hap_offsets = reinterpret_cast<const int32_t*>(D + off[9]);
+    dv.num_reads_per_group = reinterpret_cast<const int32_t*>(D + off[10]);
+    dv.num_haps_per_group = reinterpret_cast<const int32_t*>(D + off[11]);
+    dv.reads_prefix = reinterpret_cast<const int32_t*>(D + off[12]);
+    dv.haps_prefix = reinterpret_cast<const int32_t*>(D + off[13]);
+    db->eng = h;
+    db->view = dv;
+    *out = db;
+    return PHMM_OK;
+}
+
+extern "C" int phmm_device_batch_view(const phmm_device_batch* db, phmm_batch_view* out)
+{
+    if (!db || !out)
+        return phmm::fail(PHMM_EINVAL, "phmm_device_batch_view: null argument");
+    *out = db->view;
+    return PHMM_OK;
+}
+
+extern "C" void phmm_device_batch_free(phmm_device_batch* db)
+{
+    if (!db)
+        return;
+    if (db->mem) {
+        hipDeviceSynchronize();
+        hipFree(db->mem);
+    }
+    delete db;
+}
+
+extern "C" int phmm_device_alloc(phmm_engine* h, size_t bytes, void** out)
+{
+    if (!h || !out)
+        return phmm::fail(PHMM_EINVAL, "phmm_device_alloc: null argument");
+    hipSetDevice(h->e.device);
+    if (hipMalloc(out, std::max<size_t>(bytes, 1)) != hipSuccess) {
+        hipGetLastError();
+        *out = nullptr;
+        return phmm::fail(PHMM_ENOMEM, "phmm_device_alloc: device allocation failed");
+    }
+    return PHMM_OK;
+}
+
+extern "C" int phmm_device_free(phmm_engine* h, void* p)
+{
+    if (!h)
+        return phmm::fail(PHMM_EINVAL, "phmm_device_free: null engine");
+    if (p) {
+        hipDeviceSynchronize();
+        hipFree(p);
+    }
+    return PHMM_OK;
+}
+
+extern "C" int phmm_memcpy_d2h(phmm_engine* h, void* dst, const void* src, size_t bytes)
+{
+    if (!h || (bytes && (!dst || !src)))
+        return phmm::fail(PHMM_EINVAL, "phmm_memcpy_d2h: null argument");
+    hipSetDevice(h->e.device);
+    return h->e.check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "phmm_memcpy_d2h");
+}
+
+extern "C" int phmm_memcpy_h2d(phmm_engine* h, void* dst, const void* src, size_t bytes)
+{
+    if (!h || (bytes && (!dst || !src)))
+        return phmm::fail(PHMM_EINVAL, "phmm_memcpy_h2d: null argument");
+    hipSetDevice(h->e.device);
+    return h->e.check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "phmm_memcpy_h2d");
+}

@@ -1,0 +1,143 @@
+// Shared device-side definitions of the gfx950 PairHMM engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace phmm {
+
+// One alignment of the sorted work list: global read index, global hap
+// index and the read-major output slot (gpuworker.cuh:1011).
+struct __attribute__((aligned(16))) WorkItem {
+    int32_t read;
+    int32_t hap;
+    int32_t out;
+    int32_t pad;
+};
+
+// Scale of the forward variables: the reference starts row 0's deletion
+// state at FLT_MAX/16 / H and returns log10(sum) - log10(FLT_MAX/16)
+// (pairhmm_kernels.cuh:1203, 1673, 1760-1763).
+constexpr float kScale = 2.1267646664908054e37f;  // FLT_MAX / 16 (exact)
+constexpr float kLog10Scale = 37.3277194f;      // log10(FLT_MAX / 16)
+constexpr double kScaleD = 2.1267646664908054e37;
+
+// Headroom divisor of the rescaled kernel's start constant (see
+// k_fwd_tile): every rescaled cell stays below FLT_MAX/16.
+constexpr float kFastKappa = 2.0f;
+constexpr float kLog10FastKappa = 0.30103000f;
+
+constexpr int kMaxTiles = 32;
+constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
+
+// Hap length -> sort bucket: exact below 2048, then width 16.
+__host__ __device__ inline int hap_bucket(int h)
+{
+    if (h < 2048)
+        return h < 0 ? 0 : h;
+    int b = 2048 + ((h - 2048) >> 4);
+    return b < kHapBuckets - 1 ? b : kHapBuckets - 1;
+}
+
+// Gap-continuation constants of the reference GPU kernel
+// (pairhmm_kernels.cuh:1195-1196).
+constexpr float kEpsGpu = 0.1f;
+constexpr float kBetaGpu = 0.9f;
+
+// Packed read row: base code | base q << 8 | ins q << 16 | del q << 24,
+// qualities clamped to 127.
+__host__ __device__ inline int row_code(uint32_t w) { return (int)(w & 0xffu); }
+__host__ __device__ inline int row_bq(uint32_t w) { return (int)((w >> 8) & 0xffu); }
+__host__ __device__ inline int row_iq(uint32_t w) { return (int)((w >> 16) & 0xffu); }
+__host__ __device__ inline int row_dq(uint32_t w) { return (int)(w >> 24); }
+
+// A/C/G/T -> 0..3, every other byte -> 4 (utility_kernels.cuh:16-20).
+__host__ __device__ inline int base_code(uint8_t b)
+{
+    return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : 4;
+}
+
+// Per-read metadata from the read pass.
+struct __attribute__((aligned(8))) ReadMeta {
+    int32_t bin;     // work-list bin; -1: empty read
+    float nl;        // product of alpha over the read (rescaled kernel)
+};
+
+// Everything the forward kernels read.
+struct FwdParams {
+    const WorkItem* items;
+    const uint32_t* rows;        // packed read rows, indexed like read bytes
+    const uint8_t* hapc;         // hap base codes, indexed like hap bytes
+    const uint8_t* gcp;          // raw gcp qualities (CPU semantics)
+    const int32_t* read_len;
+    const int32_t* hap_len;
+    const int32_t* read_off;
+    const int32_t* hap_off;
+    const ReadMeta* meta;
+    const float* ph2pr;          // 128 entries
+    float* scores;
+    int32_t* rescue_list;        // work-item indices to recompute in FP64
+    int32_t* rescue_count;
+    float rescue_min;            // FP32 sum (reference units) threshold
+    int32_t rescue_on;
+    int32_t semantics;           // 0: reference GPU, 1: reference CPU
+    int32_t pad;
+};
+
+// Everything the binning kernels read and write (binning.hip).
+struct BinParams {
+    // batch (device pointers)
+    const uint8_t* read_data;
+    const uint8_t* hap_data;
+    const uint8_t* bq;
+    const uint8_t* iq;
+    const uint8_t* dq;
+    const uint8_t* gcp;
+    const int32_t* read_len;
+    const int32_t* hap_len;
+    const int32_t* read_off;
+    const int32_t* hap_off;
+    const int32_t* nreads;     // per group
+    const int32_t* nhaps;      // per group
+    const int32_t* rprefix;    // [G+1]
+    const int32_t* hprefix;    // [G+1]
+    int32_t G, nR, nH, NB;
+    int64_t read_bytes, hap_bytes;
+    // scratch
+    uint32_t* rows;
+    uint8_t* hapc;
+    ReadMeta* meta;
+    int32_t* gb_count;         // [G*(NB+1)]  slot NB: empty reads
+    int32_t* gb_start;         // [G*(NB+1)]
+    int32_t* read_perm;        // [nR]
+    int64_t* align_off;        // [G+1]
+    int32_t* key_count;        // [NB*kHapBuckets]
+    int32_t* key_cursor;       // [NB*kHapBuckets]
+    int32_t* bin_range;        // [2*NB]: begin, count
+    int32_t* hmax;             // [1]
+    int32_t* rescue_count;     // [1]
+    int32_t* error_flag;       // [1]
+    WorkItem* items;
+    float* scores;
+    const float* ph2pr;
+    // tiles
+    int32_t ntiles;
+    int32_t tiles[kMaxTiles];  // ascending tile sizes (lanes*rows)
+    int32_t semantics;
+    int32_t allow_fast;
+};
+
+hipError_t run_binning(const BinParams& bp, hipStream_t st);
+hipError_t launch_add_count(int64_t* total, const int32_t* count, hipStream_t st);
+hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const int32_t* range, int begin,
+                           int count, int G, int max_blocks, hipStream_t st);
+hipError_t launch_fwd_wave_f32(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
+                               const int32_t* count_ptr, const int32_t* hmax_ptr, float* spill,
+                               int64_t spill_elems, int32_t* error_flag, int blocks,
+                               hipStream_t st);
+hipError_t launch_fwd_wave_f64(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
+                               const int32_t* count_ptr, const int32_t* hmax_ptr, double* spill,
+                               int64_t spill_elems, int32_t* error_flag, int blocks,
+                               hipStream_t st);
+
+}  // namespace phmm

@@ -1,0 +1,196 @@
+"""GPU parity: the HIP engine against the CPU FP64 oracle.
+
+Bar (from the task statement): max |delta log10| <= 1e-3 against the FP64
+restatement of the reference's forward algorithm, on the same seeded inputs.
+Integer bookkeeping (output placement, chunking, modes, request mixing) must be
+bit-exact.  All calls go through the C ABI (ctypes).
+"""
+import numpy as np
+import pytest
+
+from conftest import say
+
+TOL = 1e-3  # absolute, log10
+
+pytestmark = pytest.mark.gpu
+
+
+def _max_delta(gpu, ref):
+    gpu = np.asarray(gpu, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    both_inf = np.isneginf(gpu) & np.isneginf(ref)
+    d = np.abs(gpu - ref)
+    d[both_inf] = 0.0
+    return float(np.max(d)) if d.size else 0.0
+
+
+def test_readme_known_answer(engine, oracle):
+    import gpupairhmm_amd as G
+    say("case: README record on the GPU")
+    with open("tests/golden/readme_record.txt") as f:
+        b = G.HostBatch.parse(f.read())
+    got = engine.score_batch(b)
+    expected = np.array([-5.97153, -3.1966, -6.34043, -1.66333])
+    decimals = np.array([5, 4, 5, 5])
+    assert np.all(np.abs(got - expected) <= 0.5 * 10.0 ** (-decimals) + 2e-6), got
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    assert _max_delta(got, ref) <= 2e-5
+    text = G.format_scores(b, got)
+    assert text.splitlines()[0] == "2 2"
+
+
+@pytest.mark.parametrize("config,records", [(0, 1000), (1, 40), (2, 3), (3, 4), (4, 3)])
+def test_config_parity(engine, oracle, config, records):
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(config, seed=1000 + config, num_records=records)
+    say(f"case: config {config}, {records} records, {b.num_alignments} alignments")
+    got = engine.score_batch(b)
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    d = _max_delta(got, ref)
+    print(f"config {config}: max |delta log10| = {d:.3e}")
+    assert np.all(np.isfinite(got))
+    assert d <= TOL
+
+
+def test_plain_and_batched_identical(engine, plain_engine):
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(2, seed=77, num_records=4)
+    say(f"case: plain vs batched, {b.num_alignments} alignments")
+    a = engine.score_batch(b)
+    p = plain_engine.score_batch(b)
+    assert np.array_equal(a.view(np.uint32), p.view(np.uint32))
+
+
+@pytest.mark.parametrize("tb", [1, 7, 1000, 0])
+def test_transfer_chunking_does_not_change_scores(lib, engine, tb):
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(1, seed=5, num_records=12)
+    say(f"case: transfer_batchsize {tb}")
+    base = engine.score_batch(b)
+    with G.Engine(transfer_batchsize=tb) as e2:
+        got = e2.score_batch(b)
+    assert np.array_equal(base.view(np.uint32), got.view(np.uint32))
+
+
+def _record(reads, haps, q=30, iq=40, dq=40, gcp=10):
+    """Line-record text for reads (strings) x haps (strings)."""
+    lines = [f"{len(reads)} {len(haps)}"]
+    for r in reads:
+        L = len(r)
+        bq = "".join(chr(33 + (q if isinstance(q, int) else q[i % len(q)])) for i in range(L))
+        ins = chr(33 + iq) * max(L - 1, 0) + (chr(33 + 45) if L else "")
+        dl = chr(33 + dq) * max(L - 1, 0) + (chr(33 + 45) if L else "")
+        lines.append(" ".join([r, bq, ins, dl, chr(33 + gcp) * L]))
+    lines.extend(haps)
+    return "\n".join(lines) + "\n"
+
+
+def test_edge_cases(engine, oracle):
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(3)
+    rs = lambda n: "".join(rng.choice(list("ACGT"), n)) if n else ""
+    recs = []
+    # tile boundaries of the default table and reads around them
+    tiles = [a * b for a, b in engine.tiles()]
+    lens = sorted({1, 2, 3, 5, 17} | {t + d for t in tiles for d in (-2, -1, 0) if t + d > 0})
+    for L in lens:
+        recs.append(_record([rs(L)], [rs(max(L, 40)), rs(7)]))
+    # wildcards and non-ACGT bytes on both sides, hap of length 1
+    recs.append(_record(["ACGNNTACGT", "NNNN", "AC-T*GX"], ["ACGTACGTAC", "N", "acgtn", "A"]))
+    # multi-tile reads (longer than the largest tile)
+    recs.append(_record([rs(1500), rs(1100)], [rs(1600), rs(900)]))
+    b = G.HostBatch.parse("".join(recs))
+    say(f"case: edge cases, {b.num_alignments} alignments, {len(recs)} records")
+    got = engine.score_batch(b)
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    assert _max_delta(got, ref) <= TOL
+    st = engine.stats()
+    assert st.multitile_alignments >= 0
+
+
+def test_empty_reads_and_haps(engine):
+    import gpupairhmm_amd as G
+    text = "2 2\nACGT IIII IIIN IIIN ++++\n    \nACGTAC\n\n"
+    # an empty read line is "    " (5 empty fields): length 0
+    b = G.HostBatch.parse(text)
+    say("case: empty read and empty hap")
+    got = engine.score_batch(b)
+    assert got.shape == (4,)
+    assert np.isfinite(got[0])          # read 0 x hap 0
+    assert np.isneginf(got[1])          # read 0 x empty hap
+    assert np.isneginf(got[2]) and np.isneginf(got[3])  # empty read
+
+
+def test_fp64_rescue(lib, oracle):
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(11)
+    hap = "".join(rng.choice(list("ACGT"), 3000))
+    # low base quality and a random read against the hap: log10 likelihood
+    # far below the FP32 range of the reference's scaling (~ -75)
+    reads = ["".join(rng.choice(list("ACGT"), 300)) for _ in range(3)]
+    text = _record(reads, [hap], q=[3, 4, 5, 6])
+    b = G.HostBatch.parse(text)
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    say(f"case: FP64 rescue, oracle scores {ref}")
+    assert np.all(ref < -80)
+    with G.Engine(fp64_rescue=True) as e:
+        got = e.score_batch(b)
+        st = e.stats()
+    assert st.rescued == len(reads)
+    assert _max_delta(got, ref) <= TOL
+    with G.Engine(fp64_rescue=False) as e:
+        raw = e.score_batch(b)
+    assert _max_delta(raw, ref) > TOL  # FP32 alone cannot represent these
+
+
+def test_device_api_matches_host_api(engine):
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(1, seed=9, num_records=6)
+    say("case: device-resident batch")
+    host = engine.score_batch(b)
+    db = engine.upload(b)
+    dev = engine.score_resident(db)
+    db.free()
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+
+
+def test_server_alone_equals_mix(engine):
+    import gpupairhmm_amd as G
+    reqs = [G.HostBatch.synthetic(4, seed=500 + i, num_records=1) for i in range(6)]
+    say("case: serving alone vs mix")
+    alone = [engine.score_batch(r) for r in reqs]
+    srv = G.Server(engine)
+    try:
+        for i, r in enumerate(reqs):
+            assert srv.submit(r, i)
+        got = {}
+        while len(got) < len(reqs):
+            res = srv.poll(timeout_ms=20000)
+            assert res is not None, "server timed out"
+            tag, scores, lat = res
+            got[tag] = scores
+            assert lat >= 0
+        st = srv.stats()
+    finally:
+        srv.close()
+    for i in range(len(reqs)):
+        assert np.array_equal(alone[i].view(np.uint32), got[i].view(np.uint32))
+    assert st["completed"] == len(reqs)
+
+
+def test_server_budget_refusal(engine):
+    import gpupairhmm_amd as G
+    r = G.HostBatch.synthetic(4, seed=1, num_records=1)
+    srv = G.Server(engine, hbm_budget_bytes=1 << 40, max_inflight=1)
+    say("case: serving admission limit")
+    try:
+        assert srv.submit(r, 1)
+        refused = not srv.submit(r, 2)
+        res = srv.poll(timeout_ms=20000)
+        assert res is not None
+        if refused:
+            assert srv.stats()["refused"] >= 1
+        assert srv.submit(r, 3)  # capacity is back after completion
+        assert srv.poll(timeout_ms=20000) is not None
+    finally:
+        srv.close()
