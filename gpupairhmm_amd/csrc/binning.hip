@@ -105,7 +105,7 @@ __global__ void k_read_meta(BinParams bp)
         pdelta = delta;
         palpha = alpha;
     }
-    if (ok && !(nl >= 1e-3))
+    if (ok && !(nl >= 1e-30))  // the start constant must stay a normal float
         ok = false;
     m.nl = ok ? (float)nl : 1.0f;
     m.bin = 2 * k + (ok ? 0 : 1);
@@ -304,15 +304,20 @@ __global__ __launch_bounds__(256) void k_scatter(BinParams bp)
     }
 }
 
-__global__ void k_add_count(int64_t* total, const int32_t* count)
+__global__ void k_add_count(int64_t* total, const int32_t* counts, int n)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        *total += *count;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int64_t s = 0;
+        for (int i = 0; i < n; ++i)
+            s += counts[i];
+        *total += s;
+    }
 }
 
-hipError_t launch_add_count(int64_t* total, const int32_t* count, hipStream_t st)
+// *total += sum of counts[0..n)
+hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, st, total, count);
+    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, st, total, counts, n);
     return hipGetLastError();
 }
 
@@ -328,7 +333,8 @@ hipError_t run_binning(const BinParams& bp, hipStream_t st)
     hipLaunchKernelGGL(k_clear, dim3(blocks_for((int64_t)bp.NB * kHapBuckets, tpb)), dim3(tpb), 0,
                        st, bp.key_count, (int64_t)bp.NB * kHapBuckets);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.hmax, (int64_t)1);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.rescue_count, (int64_t)1);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
     if (bp.read_bytes > 0)
         hipLaunchKernelGGL(k_encode_reads, dim3(blocks_for(bp.read_bytes, tpb)), dim3(tpb), 0, st, bp);
     if (bp.hap_bytes > 0)

@@ -64,7 +64,7 @@ Engine::~Engine()
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
     for (DevBuf* b : {&rows, &hapc, &meta, &gb_count, &gb_start, &read_perm, &align_off, &key_count,
-                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &spill})
+                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &final_list, &spill})
         if (b->p)
             hipFree(b->p);
     for (auto& s : slots)
@@ -188,18 +188,22 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(key_count, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
         (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
         (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NB, "bin ranges")) ||
-        (rc = ensure_zeroed(smallints, 256, "counters")) ||
+        (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
-        (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")))
+        (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")) ||
+        (rc = ensure(final_list, sizeof(int32_t) * (size_t)nA, "final list")))
         return rc;
     const size_t spill_bytes = (size_t)256 << 20;
     if ((rc = ensure(spill, spill_bytes, "spill")))
         return rc;
 
+    // counters: [0] max hap length, [1] final-list count, [2] error flag,
+    // [4 .. 4+NB) per-bin rescue counts
     int32_t* sm = static_cast<int32_t*>(smallints.p);
     int32_t* d_hmax = sm + 0;
-    int32_t* d_rescue_count = sm + 1;
+    int32_t* d_final_count = sm + 1;
     int32_t* d_error = sm + 2;
+    int32_t* d_rescue_counts = sm + 4;
 
     BinParams bp{};
     bp.read_data = v.read_data;
@@ -233,7 +237,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
     bp.bin_range = static_cast<int32_t*>(bin_range.p);
     bp.hmax = d_hmax;
-    bp.rescue_count = d_rescue_count;
+    bp.rescue_counts = d_rescue_counts;
+    bp.final_count = d_final_count;
     bp.error_flag = d_error;
     bp.items = static_cast<WorkItem*>(items.p);
     bp.scores = d_scores;
@@ -259,7 +264,9 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     fp.ph2pr = d_ph2pr;
     fp.scores = d_scores;
     fp.rescue_list = static_cast<int32_t*>(rescue_list.p);
-    fp.rescue_count = d_rescue_count;
+    fp.rescue_counts = d_rescue_counts;
+    fp.final_list = static_cast<int32_t*>(final_list.p);
+    fp.final_count = d_final_count;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;
@@ -277,6 +284,12 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         }
     }
     const int persist_blocks = num_cus * 8;
+    const int mb = 2 * ntiles;  // multi-tile bin
+    float* spill32 = static_cast<float*>(spill.p);
+    double* spill64 = static_cast<double*>(spill.p);
+    const int64_t n32 = (int64_t)(spill.cap / sizeof(float)), n64 = (int64_t)(spill.cap / sizeof(double));
+    // pass 1: FP32 tile kernels per bin (results below the rescue threshold
+    // go to the bin's rescue list), FP32 multi-tile kernel for long reads
     if (opt.mode == PHMM_MODE_PLAIN) {
         // one host round trip per batch for the per-bin sizes (the reference
         // syncs per chunk for its per-partition counts, gpuworker.cuh:852-854)
@@ -288,24 +301,25 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         if (timed)
             hipEventRecord(evp.first, st);
         for (int b = 0; b < 2 * ntiles; ++b) {
-            const int begin = h_bin_range[2 * b], count = h_bin_range[2 * b + 1];
+            const int count = h_bin_range[2 * b + 1];
             if (count <= 0)
                 continue;
             const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, nullptr, begin, count,
-                                            tc.lanes, 0, st),
-                            "forward kernel")))
+            TileLaunch tl;
+            tl.begin = h_bin_range[2 * b];
+            tl.count = count;
+            tl.G = tc.lanes;
+            tl.bin = b;
+            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
                 return rc;
             if (b % 2 == 0)
                 stats.fast_alignments += count;
         }
-        const int mb = 2 * ntiles;
         if (h_bin_range[2 * mb + 1] > 0) {
             stats.multitile_alignments += h_bin_range[2 * mb + 1];
             if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
-                                                bp.bin_range + 2 * mb + 1, d_hmax,
-                                                static_cast<float*>(spill.p),
-                                                (int64_t)(spill.cap / sizeof(float)), d_error,
+                                                bp.bin_range + 2 * mb + 1, d_hmax, spill32, n32,
+                                                d_error,
                                                 std::min(persist_blocks, h_bin_range[2 * mb + 1]),
                                                 st),
                             "multi-tile kernel")))
@@ -316,17 +330,18 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
             hipEventRecord(evp.first, st);
         for (int b = 0; b < 2 * ntiles; ++b) {
             const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, bp.bin_range + 2 * b, 0, 0,
-                                            tc.lanes, persist_blocks, st),
-                            "forward kernel")))
+            TileLaunch tl;
+            tl.begin_ptr = bp.bin_range + 2 * b;
+            tl.count_ptr = bp.bin_range + 2 * b + 1;
+            tl.G = tc.lanes;
+            tl.bin = b;
+            tl.max_blocks = persist_blocks;
+            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
                 return rc;
         }
-        const int mb = 2 * ntiles;
         if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
-                                            bp.bin_range + 2 * mb + 1, d_hmax,
-                                            static_cast<float*>(spill.p),
-                                            (int64_t)(spill.cap / sizeof(float)), d_error,
-                                            persist_blocks, st),
+                                            bp.bin_range + 2 * mb + 1, d_hmax, spill32, n32,
+                                            d_error, persist_blocks, st),
                         "multi-tile kernel")))
             return rc;
     }
@@ -334,13 +349,39 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         hipEventRecord(evp.second, st);
         ev_pending.push_back(evp);
     }
+    // pass 2: FP64 recompute of each bin's rescue list with an FP64 tile of
+    // at least the bin's size (tiles above 512 rows: one wave per alignment);
+    // pass 3: FP64 waves for anything still not finished
     if (opt.fp64_rescue) {
-        if ((rc = check(launch_fwd_wave_f64(fp, fp.rescue_list, nullptr, d_rescue_count, d_hmax,
-                                            static_cast<double*>(spill.p),
-                                            (int64_t)(spill.cap / sizeof(double)), d_error,
-                                            persist_blocks, st),
-                        "FP64 rescue kernel")) ||
-            (rc = check(launch_add_count(d_rescued_total, d_rescue_count, st), "rescue count")))
+        for (int b = 0; b < 2 * ntiles; ++b) {
+            if (opt.mode == PHMM_MODE_PLAIN && h_bin_range[2 * b + 1] <= 0)
+                continue;
+            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
+            const int tile = tc.lanes * tc.rows;
+            if (tile <= 512) {
+                const int nr64 = tile <= 64 ? 4 : 8;
+                TileLaunch tl;
+                tl.list = fp.rescue_list;
+                tl.begin_ptr = bp.bin_range + 2 * b;
+                tl.count_ptr = d_rescue_counts + b;
+                tl.G = (tile + nr64 - 1) / nr64;
+                tl.bin = b;
+                tl.max_blocks = num_cus * 2;
+                if ((rc = check(launch_fwd_tile_f64(nr64, fp, tl, st), "FP64 tile kernel")))
+                    return rc;
+            } else if ((rc = check(launch_fwd_wave_f64(fp, fp.rescue_list, bp.bin_range + 2 * b,
+                                                       d_rescue_counts + b, d_hmax, spill64, n64,
+                                                       d_error, num_cus * 2, st),
+                                   "FP64 wave kernel"))) {
+                return rc;
+            }
+        }
+        if ((rc = check(launch_fwd_wave_f64(fp, fp.final_list, nullptr, d_final_count, d_hmax,
+                                            spill64, n64, d_error, num_cus * 2, st),
+                        "FP64 final kernel")) ||
+            (rc = check(launch_add_count(d_rescued_total, d_rescue_counts, 2 * ntiles, st),
+                        "rescue count")) ||
+            (rc = check(launch_add_count(d_rescued_total, d_final_count, 1, st), "rescue count")))
             return rc;
     }
     stats.alignments += nA;

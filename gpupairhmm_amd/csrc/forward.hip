@@ -31,7 +31,7 @@
 //    e_r = eps delta_{r-1}/(delta_r alpha_{r-1}).  The start constant is
 //    lowered by n_R/2 (it cancels in the score) so that every rescaled value
 //    stays below FLT_MAX/16; the read pass admits a read only when
-//    0 < alpha <= 1, e_r <= 1/2, eps <= 1/2 and n_R >= 1e-3.
+//    0 < alpha <= 1, e_r <= 1/2, eps <= 1/2 and n_R >= 1e-30.
 //
 // Both forms use the reference's out-of-bounds-row trick for the result
 // (pairhmm_kernels.cuh:1325-1356): the first row past the read turns the
@@ -80,12 +80,53 @@ __device__ __forceinline__ float ph2pr_of(const float* table, int q)
 // ---------------------------------------------------------------------------
 // k_fwd_tile
 // ---------------------------------------------------------------------------
-// One task: the W = 64/G alignments items[begin + task*W ...] of a bin.
-template <int NR, bool FAST>
-__device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, int begin,
-                                              int count, int G, int task)
+template <typename T>
+struct TileTraits;
+template <>
+struct TileTraits<float> {
+    using V = float4;  // one ds_read_b128 = 4 rows
+    static constexpr int K = 4;
+    __device__ static V pack(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
+    __device__ static void unpack(const V& x, float* v)
+    {
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+    }
+    __device__ static float shr(float x) { return shr1f(x); }
+};
+template <>
+struct TileTraits<double> {
+    using V = double2;  // one ds_read_b128 = 2 rows
+    static constexpr int K = 2;
+    __device__ static V pack(const double* v) { return make_double2(v[0], v[1]); }
+    __device__ static void unpack(const V& x, double* v)
+    {
+        v[0] = x.x;
+        v[1] = x.y;
+    }
+    __device__ static double shr(double x) { return shr1d(x); }
+};
+
+// One task: W = 64/G alignments of a bin.  Item k of the bin is
+// items[list ? list[begin + k] : begin + k].
+//
+// T = float is the first pass: results whose FP32 sum (in the reference's
+// units) is below rescue_min are appended to the bin's rescue list
+// (rescue_list[begin + n], n counted in rescue_counts[bin]).  T = double is
+// the FP64 recompute of those lists (start constant 2^1000/H); a result that
+// is still not a positive finite number goes to the final list.
+template <typename T, int NR, bool FAST>
+__device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileTraits<T>::V* tab,
+                                              const int32_t* list, int begin, int count, int G,
+                                              int task, int bin)
 {
-    constexpr int NQ = NR / 4;
+    using TT = TileTraits<T>;
+    using V = typename TT::V;
+    constexpr int K = TT::K;
+    constexpr int NQ = NR / K;
+    constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     const int W = 64 / G;
     const int grp = lane / G;
@@ -97,7 +138,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, i
     int L = 0, HL = 0, roff = 0, hoff = 0, out = -1, idx = -1;
     float nl = 1.0f;
     if (active) {
-        idx = begin + local;
+        idx = list ? list[begin + local] : begin + local;
         const WorkItem it = p.items[idx];
         out = it.out;
         L = p.read_len[it.read];
@@ -107,48 +148,49 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, i
         if (FAST)
             nl = p.meta[it.read].nl;
     }
-    float eps = kEpsGpu, beta = kBetaGpu;
+    T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
     if (p.semantics == 1 && active && L > 0) {
-        eps = ph2pr_of(p.ph2pr, p.gcp[roff]);
-        beta = (float)(1.0 - (double)eps);
+        const float e = ph2pr_of(p.ph2pr, p.gcp[roff]);
+        eps = (T)e;
+        beta = F32 ? (T)(float)(1.0 - (double)e) : (T)1 - (T)e;
     }
 
     // ---- per-row coefficients and the emission table -------------------
     const int r0 = t * NR;
-    float c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e -; SAFE: alpha w delta' e
+    T c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e -; SAFE: alpha w delta' e
     {
-        float pdelta = 0.f, psigma = 0.f, palpha = 1.f;
+        T pdelta = 0, psigma = 0, palpha = 1;
         if (r0 >= 1 && r0 - 1 < L) {
             const uint32_t w = p.rows[roff + r0 - 1];
-            pdelta = ph2pr_of(p.ph2pr, row_iq(w));
-            psigma = ph2pr_of(p.ph2pr, row_dq(w));
-            palpha = 1.0f - (pdelta + psigma);
+            pdelta = (T)ph2pr_of(p.ph2pr, row_iq(w));
+            psigma = (T)ph2pr_of(p.ph2pr, row_dq(w));
+            palpha = (T)1 - (pdelta + psigma);
         }
-        float lm[NR], lx[NR];
+        T lm[NR], lx[NR];
         int code[NR];
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             const int ri = r0 + j;  // 0-based row; 1-based row ri+1
             if (ri < L) {
                 const uint32_t w = p.rows[roff + ri];
-                const float pb = ph2pr_of(p.ph2pr, row_bq(w));
-                const float delta = ph2pr_of(p.ph2pr, row_iq(w));
-                const float sigma = ph2pr_of(p.ph2pr, row_dq(w));
-                const float alpha = 1.0f - (delta + sigma);
+                const T pb = (T)ph2pr_of(p.ph2pr, row_bq(w));
+                const T delta = (T)ph2pr_of(p.ph2pr, row_iq(w));
+                const T sigma = (T)ph2pr_of(p.ph2pr, row_dq(w));
+                const T alpha = (T)1 - (delta + sigma);
                 code[j] = row_code(w);
-                lm[j] = 1.0f - pb;
-                lx[j] = pb / 3.0f;
+                lm[j] = (T)1 - pb;
+                lx[j] = pb / (T)3;
                 if (FAST) {
                     if (ri == 0) {
-                        c0[j] = 0.f;
+                        c0[j] = 0;
                         c1[j] = beta / alpha;
-                        c2[j] = 0.f;
+                        c2[j] = 0;
                     } else {
                         c0[j] = beta * pdelta / (palpha * alpha);
                         c1[j] = beta * psigma / alpha;
                         c2[j] = eps * pdelta / (delta * palpha);
                     }
-                    c3[j] = 0.f;
+                    c3[j] = 0;
                 } else {
                     c0[j] = alpha;
                     c1[j] = ri == 0 ? beta : beta * psigma;
@@ -160,53 +202,57 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, i
                 palpha = alpha;
             } else {
                 code[j] = 4;
-                lm[j] = 0.f;
-                lx[j] = 0.f;
+                lm[j] = 0;
+                lx[j] = 0;
                 const bool firstoob = (ri == L);
                 if (FAST) {
-                    c0[j] = 0.f;
-                    c1[j] = 0.f;
-                    c2[j] = firstoob ? pdelta / palpha : 1.0f;
-                    c3[j] = 0.f;
+                    c0[j] = 0;
+                    c1[j] = 0;
+                    c2[j] = firstoob ? pdelta / palpha : (T)1;
+                    c3[j] = 0;
                 } else {
-                    c0[j] = 0.f;
-                    c1[j] = 0.f;
-                    c2[j] = firstoob ? beta : 0.f;
-                    c3[j] = 1.0f;
+                    c0[j] = 0;
+                    c1[j] = 0;
+                    c2[j] = firstoob ? beta : (T)0;
+                    c3[j] = 1;
                 }
             }
         }
-        // emission table: tab[(letter*NQ + q)*64 + lane] = rows 4q..4q+3
+        // emission table: tab[(letter*NQ + q)*64 + lane] = rows K*q .. K*q+K-1
 #pragma unroll
         for (int l = 0; l < 5; ++l) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                float v[4];
+                T v[K];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int j = 4 * q + k;
+                for (int k = 0; k < K; ++k) {
+                    const int j = K * q + k;
                     const bool m = (l == code[j]) || (l == 4) || (code[j] == 4);
                     v[k] = m ? lm[j] : lx[j];
                 }
-                tab[(l * NQ + q) * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
+                tab[(l * NQ + q) * 64 + lane] = TT::pack(v);
             }
         }
     }
 
     // ---- wavefront sweep -------------------------------------------------
-    float M[NR], D[NR];
+    T M[NR], D[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
-        M[j] = 0.f;
-        D[j] = 0.f;
+        M[j] = 0;
+        D[j] = 0;
     }
-    float C0 = 0.f;
-    if (active && HL > 0)
-        C0 = FAST ? (kScale / (float)HL) * (nl * (1.0f / kFastKappa)) : kScale / (float)HL;
-    float Ml = 0.f, Dl = first ? C0 : 0.f;
-    float Mul = 0.f, Dul = Dl;
-    float Iin = 0.f;
-    float acc = 0.f;
+    T C0 = 0;
+    if (active && HL > 0) {
+        if (F32)
+            C0 = FAST ? (T)((kScale / (float)HL) * (nl * (1.0f / kFastKappa))) : (T)(kScale / (float)HL);
+        else
+            C0 = (T)(kStartF64 / (double)HL);
+    }
+    T Ml = 0, Dl = first ? C0 : (T)0;
+    T Mul = 0, Dul = Dl;
+    T Iin = 0;
+    T acc = 0;
     int letter = 4;
 
     const int Hmax = wave_max(HL);
@@ -221,42 +267,42 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, i
         for (int u = 0; u < 4; ++u) {
             const int sh = shr1i(letter);
             letter = first ? (int)((wcur >> (8 * u)) & 0xffu) : sh;
-            const float4* trow = tab + letter * (NQ * 64) + lane;
-            float mp = Mul, dp = Dul, chain = Iin;
+            const V* trow = tab + letter * (NQ * 64) + lane;
+            T mp = Mul, dp = Dul, chain = Iin;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const float4 lam4 = trow[q * 64];
-                const float lam[4] = {lam4.x, lam4.y, lam4.z, lam4.w};
+                T lam[K];
+                TT::unpack(trow[q * 64], lam);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int j = 4 * q + k;
-                    const float mo = M[j], dov = D[j];
-                    float t1;
+                for (int k = 0; k < K; ++k) {
+                    const int j = K * q + k;
+                    const T mo = M[j], dov = D[j];
+                    T t1;
                     if (FAST) {
-                        t1 = fmaf(c1[j], dp, mp);
-                        t1 = fmaf(c0[j], chain, t1);
-                        chain = fmaf(c2[j], chain, mp);
+                        t1 = fma(c1[j], dp, mp);
+                        t1 = fma(c0[j], chain, t1);
+                        chain = fma(c2[j], chain, mp);
                     } else {
-                        t1 = fmaf(c1[j], dp, chain);
-                        t1 = fmaf(c0[j], mp, t1);
-                        chain = fmaf(c2[j], mp, c3[j] * chain);
+                        t1 = fma(c1[j], dp, chain);
+                        t1 = fma(c0[j], mp, t1);
+                        chain = fma(c2[j], mp, c3[j] * chain);
                     }
                     M[j] = lam[k] * t1;
-                    D[j] = fmaf(eps, dov, mo);
+                    D[j] = fma(eps, dov, mo);
                     mp = mo;
                     dp = dov;
                 }
             }
             const int c = s4 + u - t;
-            acc += (c >= 1 && c <= HL) ? chain : 0.f;
-            const float sM = shr1f(M[NR - 1]);
-            const float sD = shr1f(D[NR - 1]);
-            const float sI = shr1f(chain);
+            acc += (c >= 1 && c <= HL) ? chain : (T)0;
+            const T sM = TT::shr(M[NR - 1]);
+            const T sD = TT::shr(D[NR - 1]);
+            const T sI = TT::shr(chain);
             Mul = Ml;
             Dul = Dl;
-            Ml = first ? 0.f : sM;
+            Ml = first ? (T)0 : sM;
             Dl = first ? C0 : sD;
-            Iin = first ? 0.f : sI;
+            Iin = first ? (T)0 : sI;
         }
         const int k = (s4 >> 2) + 2;
         wcur = wnext;
@@ -264,38 +310,50 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, float4* tab, i
     }
 
     if (active && t == L / NR) {
-        float score, raw;
-        if (FAST) {
-            score = log10f(acc) - (kLog10Scale - kLog10FastKappa);
-            raw = acc * nl;
+        if (F32) {
+            const float a = (float)acc;
+            float score, raw;
+            if (FAST) {
+                score = log10f(a) - (kLog10Scale - kLog10FastKappa);
+                raw = a * nl;
+            } else {
+                score = log10f(a) - (kLog10Scale + log10f((float)beta));
+                raw = a / (float)beta;
+            }
+            p.scores[out] = score;
+            if (p.rescue_on && !(raw >= p.rescue_min)) {
+                const int n = atomicAdd(p.rescue_counts + bin, 1);
+                p.rescue_list[begin + n] = idx;
+            }
         } else {
-            score = log10f(acc) - (kLog10Scale + log10f(beta));
-            raw = acc / beta;
-        }
-        p.scores[out] = score;
-        if (p.rescue_on && !(raw >= p.rescue_min)) {
-            const int k = atomicAdd(p.rescue_count, 1);
-            p.rescue_list[k] = idx;
+            const double a = (double)acc;
+            p.scores[out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
+            if (!(a > 0.0 && a < 1e300)) {
+                const int n = atomicAdd(p.final_count, 1);
+                p.final_list[n] = idx;
+            }
         }
     }
 }
 
-// Bin range from `range` (device {begin, count}) when non-null, else the
-// host values; blocks loop over tasks so a fixed grid covers any count.
-template <int NR, bool FAST>
-__global__ __launch_bounds__(64) void k_fwd_tile(FwdParams p, const int32_t* range, int begin,
-                                                 int count, int G)
+// Blocks loop over the tasks of one bin, so a fixed grid covers any count.
+// begin/count come from device memory when the pointers are non-null.
+template <typename T, int NR, bool FAST>
+__global__ __launch_bounds__(64) void k_fwd_tile(FwdParams p, const int32_t* list,
+                                                 const int32_t* begin_ptr,
+                                                 const int32_t* count_ptr, int begin, int count,
+                                                 int G, int bin)
 {
-    static_assert(NR % 4 == 0 && NR >= 4 && NR <= 32, "rows per lane");
-    __shared__ float4 tab[5 * (NR / 4) * 64];
-    if (range) {
-        begin = range[0];
-        count = range[1];
-    }
+    static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
+    __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
+    if (begin_ptr)
+        begin = *begin_ptr;
+    if (count_ptr)
+        count = *count_ptr;
     const int W = 64 / G;
     const int ntasks = (count + W - 1) / W;
     for (int task = blockIdx.x; task < ntasks; task += gridDim.x)
-        fwd_tile_task<NR, FAST>(p, tab, begin, count, G, task);
+        fwd_tile_task<T, NR, FAST>(p, tab, list, begin, count, G, task, bin);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,17 +364,30 @@ struct WaveScalar;
 template <>
 struct WaveScalar<float> {
     __device__ static float shr(float v) { return shr1f(v); }
-    __device__ static float scale() { return kScale; }
+    __device__ static float ldexp(float v, int k) { return ldexpf(v, k); }
+    __device__ static int ilogb(float v) { return ilogbf(v); }
 };
 template <>
 struct WaveScalar<double> {
     __device__ static double shr(double v) { return shr1d(v); }
-    __device__ static double scale() { return kScaleD; }
+    __device__ static double ldexp(double v, int k) { return ::ldexp(v, k); }
+    __device__ static int ilogb(double v) { return ::ilogb(v); }
 };
 
-// Work source: items [*begin_ptr, *begin_ptr + *count_ptr) when list is
-// null, else the item indices list[0 .. *count_ptr).  `spill` holds
-// spill_elems values of T; each resident block uses 3*(hmax+1) of them.
+// Item w of the work is items[list ? list[begin + w] : begin + w] for
+// w < *count_ptr, begin = begin_ptr ? *begin_ptr : 0.  `spill` holds
+// spill_elems values of T; each resident block holds the border row's M, I,
+// D per column and the power-of-two exponents they were stored with.
+//
+// Values carry a per-lane power-of-two exponent `sc` (true = stored * 2^sc).
+// Before a lane takes in values from the lane above (or, for lane 0 of a
+// lower tile, from the spilled border row) it compares their exponent, in
+// its own scale, with its own largest value and rescales itself when the
+// larger of the two leaves [2^-32, 2^96]; so nothing it receives can
+// overflow or vanish, and the result is finite however low the score.
+// T = float (multi-tile first pass) appends results below the rescue
+// threshold (as the tile kernels define it) to the final list, which the
+// T = double instance recomputes.
 template <typename T, int NR>
 __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* list,
                                                  const int32_t* begin_ptr,
@@ -326,11 +397,15 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
 {
     using S = WaveScalar<T>;
     constexpr int TR = 64 * NR;
+    constexpr int kNone = -(1 << 28);
     const int lane = threadIdx.x;
     const int n = *count_ptr;
-    const int begin = list ? 0 : *begin_ptr;
+    const int begin = begin_ptr ? *begin_ptr : 0;
     const int hmax = *hmax_ptr;
-    const int64_t stride = 3 * (int64_t)(hmax + 1);
+    // per block: 3*(hmax+1) values of T, then 2*(hmax+1) int32 exponents,
+    // in units of T (rounded up to 16 bytes)
+    const int64_t bytes = 3 * (int64_t)(hmax + 1) * (int64_t)sizeof(T) + 8 * (int64_t)(hmax + 1);
+    const int64_t stride = ((bytes + 15) / 16) * 16 / (int64_t)sizeof(T);
     const int64_t cap_blocks = spill_elems / stride;
     const int nblk = (int)(cap_blocks < (int64_t)gridDim.x ? cap_blocks : (int64_t)gridDim.x);
     if (nblk <= 0) {
@@ -341,9 +416,12 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
     if ((int)blockIdx.x >= nblk)
         return;
     T* sp = spill + (int64_t)blockIdx.x * stride;
+    // per column c: sp[3c+0] = M, sp[3c+1] = I, sp[3c+2] = D of the border
+    // row; ex[2c] / ex[2c+1] the exponents M/D and I were stored with
+    int32_t* ex = reinterpret_cast<int32_t*>(sp + 3 * (int64_t)(hmax + 1));
 
     for (int w = blockIdx.x; w < n; w += nblk) {
-        const int idx = list ? list[w] : begin + w;
+        const int idx = list ? list[begin + w] : begin + w;
         const WorkItem it = p.items[idx];
         const int L = p.read_len[it.read];
         const int HL = p.hap_len[it.hap];
@@ -367,9 +445,10 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
             eps = (T)e;
             beta = (T)1 - (T)e;
         }
-        const T C0 = S::scale() / (T)HL;
+        const T C0 = S::ldexp((T)1, 64) / (T)HL;
         const int ntiles = (L + 1 + TR - 1) / TR;
         T acc = 0;
+        int sc = 0;
         for (int k = 0; k < ntiles; ++k) {
             const int r0 = k * TR + lane * NR;
             T ca[NR], cdl[NR], cs[NR], ce[NR], lm[NR], lx[NR];
@@ -406,20 +485,60 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
                 D[j] = 0;
             }
             const bool top = (k == 0);
+            sc = 0;
+            acc = 0;
+            T C0s = C0;
             T Ml = 0, Dl = (lane == 0 && top) ? C0 : (T)0, Mul = 0, Dul = Dl, Iin = 0;
             const bool last_tile = (k == ntiles - 1);
             const int tr = last_tile ? (L - k * TR) / NR : -1;
             const int nsteps = HL + 64;
+            auto rescale = [&](int kk) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    M[j] = S::ldexp(M[j], kk);
+                    D[j] = S::ldexp(D[j], kk);
+                }
+                Ml = S::ldexp(Ml, kk);
+                Dl = S::ldexp(Dl, kk);
+                Mul = S::ldexp(Mul, kk);
+                Dul = S::ldexp(Dul, kk);
+                Iin = S::ldexp(Iin, kk);
+                acc = S::ldexp(acc, kk);
+                C0s = S::ldexp(C0s, kk);
+                sc -= kk;
+            };
+            // make room for incoming values of exponent e_in (own scale)
+            auto fit = [&](int e_in) {
+                T my = fmax(fmax(fmax(Ml, Dl), fmax(Mul, Dul)), fmax(Iin, acc));
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+                    my = fmax(my, fmax(M[j], D[j]));
+                const int e_my = my > (T)0 ? S::ilogb(my) : kNone;
+                const int e = e_in > e_my ? e_in : e_my;
+                if (e > kNone && (e > 96 || e < -32))
+                    rescale(64 - e);
+            };
+            auto expo = [&](T v, int e) { return v > (T)0 ? S::ilogb(v) + e : kNone; };
             for (int s = 0; s < nsteps; ++s) {
                 const int c = s - lane;  // this lane's 0-based column
                 const int letter = (c >= 0 && c < HL) ? (int)p.hapc[hoff + c] : 4;
                 T mp = Mul, dp = Dul, chain = Iin;
                 if (lane == 0 && !top) {
                     // row above the tile at column c-1, from the previous tile
-                    if (c >= 1) {
-                        mp = sp[3 * (c - 1) + 0];
-                        chain = sp[3 * (c - 1) + 1];
-                        dp = sp[3 * (c - 1) + 2];
+                    // (only columns the previous tile wrote: c-1 < HL)
+                    if (c >= 1 && c - 1 < HL) {
+                        const int emd = ex[2 * (c - 1)] - sc, ei = ex[2 * (c - 1) + 1] - sc;
+                        const T rm = sp[3 * (c - 1) + 0], ri = sp[3 * (c - 1) + 1];
+                        const T rd = sp[3 * (c - 1) + 2];
+                        int e_in = expo(rm, emd);
+                        const int e2 = expo(rd, emd), e3 = expo(ri, ei);
+                        e_in = e2 > e_in ? e2 : e_in;
+                        e_in = e3 > e_in ? e3 : e_in;
+                        fit(e_in);
+                        const int emd2 = ex[2 * (c - 1)] - sc, ei2 = ex[2 * (c - 1) + 1] - sc;
+                        mp = S::ldexp(rm, emd2);
+                        chain = S::ldexp(ri, ei2);
+                        dp = S::ldexp(rd, emd2);
                     } else {
                         mp = chain = dp = 0;
                     }
@@ -440,70 +559,104 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
                     if (c >= 0 && c < HL) {
                         sp[3 * c + 0] = M[NR - 1];
                         sp[3 * c + 2] = D[NR - 1];
+                        ex[2 * c] = sc;
                     }
-                    if (c >= 1 && c <= HL)
+                    if (c >= 1 && c <= HL) {
                         sp[3 * (c - 1) + 1] = chain;
+                        ex[2 * (c - 1) + 1] = sc;
+                    }
                 }
                 const T sM = S::shr(M[NR - 1]);
                 const T sD = S::shr(D[NR - 1]);
                 const T sI = S::shr(chain);
+                const int ssc = shr1i(sc);
+                if (lane != 0) {
+                    int e_in = expo(sM, ssc - sc);
+                    const int e2 = expo(sD, ssc - sc), e3 = expo(sI, ssc - sc);
+                    e_in = e2 > e_in ? e2 : e_in;
+                    e_in = e3 > e_in ? e3 : e_in;
+                    fit(e_in);
+                } else {
+                    fit(kNone);
+                }
+                const int d = ssc - sc;
                 Mul = Ml;
                 Dul = Dl;
-                Ml = lane == 0 ? (T)0 : sM;
-                Dl = lane == 0 ? (top ? C0 : (T)0) : sD;
-                Iin = lane == 0 ? (T)0 : sI;
+                Ml = lane == 0 ? (T)0 : S::ldexp(sM, d);
+                Dl = lane == 0 ? (top ? C0s : (T)0) : S::ldexp(sD, d);
+                Iin = lane == 0 ? (T)0 : S::ldexp(sI, d);
             }
             __syncthreads();  // spilled row visible to the next tile
         }
         // the result lane of the last tile owns acc
         const int tr = (L - (ntiles - 1) * TR) / NR;
-        if (lane == tr)
-            p.scores[it.out] = (float)(log10(acc) - log10(S::scale()));
+        if (lane == tr) {
+            const double lg = log10((double)acc) + (double)sc * 0.30102999566398119521 -
+                              log10((double)C0 * (double)HL);
+            p.scores[it.out] = (float)lg;
+            // same criterion as the tile kernels: the sum in the reference's
+            // units (start FLT_MAX/16/H) below rescue_min, or not finite
+            const double lmin = log10((double)p.rescue_min) - (double)kLog10Scale;
+            if (sizeof(T) == 4 && p.rescue_on && !(acc > (T)0 && lg >= lmin && lg < 1e30)) {
+                const int q = atomicAdd(p.final_count, 1);
+                p.final_list[q] = idx;
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int NR>
-static hipError_t launch_tile_nr(bool fast, const FwdParams& p, const int32_t* range, int begin,
-                                 int count, int G, int max_blocks, hipStream_t st)
+template <typename T, int NR>
+static hipError_t launch_tile_nr(bool fast, const FwdParams& p, const TileLaunch& tl,
+                                 hipStream_t st)
 {
-    int blocks = max_blocks;
-    if (!range) {
-        const int W = 64 / G;
-        const int need = (count + W - 1) / W;
+    int blocks = tl.max_blocks;
+    if (!tl.count_ptr) {
+        const int W = 64 / tl.G;
+        const int need = (tl.count + W - 1) / W;
         if (need <= 0)
             return hipSuccess;
-        blocks = max_blocks > 0 && max_blocks < need ? max_blocks : need;
+        blocks = tl.max_blocks > 0 && tl.max_blocks < need ? tl.max_blocks : need;
     }
     if (blocks <= 0)
         return hipErrorInvalidValue;
-    if (fast)
-        hipLaunchKernelGGL((k_fwd_tile<NR, true>), dim3(blocks), dim3(64), 0, st, p, range, begin,
-                           count, G);
+    const dim3 grid(blocks), block(64);
+    if (fast && sizeof(T) == 4)
+        hipLaunchKernelGGL((k_fwd_tile<float, NR, true>), grid, block, 0, st, p, tl.list,
+                           tl.begin_ptr, tl.count_ptr, tl.begin, tl.count, tl.G, tl.bin);
     else
-        hipLaunchKernelGGL((k_fwd_tile<NR, false>), dim3(blocks), dim3(64), 0, st, p, range, begin,
-                           count, G);
+        hipLaunchKernelGGL((k_fwd_tile<T, NR, false>), grid, block, 0, st, p, tl.list,
+                           tl.begin_ptr, tl.count_ptr, tl.begin, tl.count, tl.G, tl.bin);
     return hipGetLastError();
 }
 
-// max_blocks: grid cap (0: one block per task); with a device range the
-// grid is max_blocks and the blocks loop over the bin's tasks.
-hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const int32_t* range, int begin,
-                           int count, int G, int max_blocks, hipStream_t st)
+hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaunch& tl,
+                           hipStream_t st)
 {
-    if (G < 1 || G > 64)
+    if (tl.G < 1 || tl.G > 64)
         return hipErrorInvalidValue;
     switch (NR) {
-    case 4: return launch_tile_nr<4>(fast, p, range, begin, count, G, max_blocks, st);
-    case 8: return launch_tile_nr<8>(fast, p, range, begin, count, G, max_blocks, st);
-    case 12: return launch_tile_nr<12>(fast, p, range, begin, count, G, max_blocks, st);
-    case 16: return launch_tile_nr<16>(fast, p, range, begin, count, G, max_blocks, st);
-    case 20: return launch_tile_nr<20>(fast, p, range, begin, count, G, max_blocks, st);
-    case 24: return launch_tile_nr<24>(fast, p, range, begin, count, G, max_blocks, st);
-    case 28: return launch_tile_nr<28>(fast, p, range, begin, count, G, max_blocks, st);
-    case 32: return launch_tile_nr<32>(fast, p, range, begin, count, G, max_blocks, st);
+    case 4: return launch_tile_nr<float, 4>(fast, p, tl, st);
+    case 8: return launch_tile_nr<float, 8>(fast, p, tl, st);
+    case 12: return launch_tile_nr<float, 12>(fast, p, tl, st);
+    case 16: return launch_tile_nr<float, 16>(fast, p, tl, st);
+    case 20: return launch_tile_nr<float, 20>(fast, p, tl, st);
+    case 24: return launch_tile_nr<float, 24>(fast, p, tl, st);
+    case 28: return launch_tile_nr<float, 28>(fast, p, tl, st);
+    case 32: return launch_tile_nr<float, 32>(fast, p, tl, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fwd_tile_f64(int NR, const FwdParams& p, const TileLaunch& tl, hipStream_t st)
+{
+    if (tl.G < 1 || tl.G > 64)
+        return hipErrorInvalidValue;
+    switch (NR) {
+    case 4: return launch_tile_nr<double, 4>(false, p, tl, st);
+    case 8: return launch_tile_nr<double, 8>(false, p, tl, st);
     default: return hipErrorInvalidValue;
     }
 }

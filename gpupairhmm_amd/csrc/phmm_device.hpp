@@ -27,6 +27,11 @@ constexpr double kScaleD = 2.1267646664908054e37;
 constexpr float kFastKappa = 2.0f;
 constexpr float kLog10FastKappa = 0.30103000f;
 
+// The FP64 recompute starts at 2^1000/H: scores down to about -600 stay in
+// double's normal range.
+constexpr double kStartF64 = 0x1p1000;
+constexpr double kLog10StartF64 = 301.02999566398120;  // 1000 * log10(2)
+
 constexpr int kMaxTiles = 32;
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 
@@ -76,8 +81,10 @@ struct FwdParams {
     const ReadMeta* meta;
     const float* ph2pr;          // 128 entries
     float* scores;
-    int32_t* rescue_list;        // work-item indices to recompute in FP64
-    int32_t* rescue_count;
+    int32_t* rescue_list;        // per bin: work-item indices to recompute in FP64
+    int32_t* rescue_counts;      // per bin
+    int32_t* final_list;         // items the FP64 tile pass could not finish
+    int32_t* final_count;
     float rescue_min;            // FP32 sum (reference units) threshold
     int32_t rescue_on;
     int32_t semantics;           // 0: reference GPU, 1: reference CPU
@@ -115,7 +122,8 @@ struct BinParams {
     int32_t* key_cursor;       // [NB*kHapBuckets]
     int32_t* bin_range;        // [2*NB]: begin, count
     int32_t* hmax;             // [1]
-    int32_t* rescue_count;     // [1]
+    int32_t* rescue_counts;    // [NB]
+    int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
     WorkItem* items;
     float* scores;
@@ -128,9 +136,20 @@ struct BinParams {
 };
 
 hipError_t run_binning(const BinParams& bp, hipStream_t st);
-hipError_t launch_add_count(int64_t* total, const int32_t* count, hipStream_t st);
-hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const int32_t* range, int begin,
-                           int count, int G, int max_blocks, hipStream_t st);
+hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
+// One launch of k_fwd_tile over one bin (or one bin's rescue list).
+struct TileLaunch {
+    const int32_t* list = nullptr;       // item indices, or null: items are contiguous
+    const int32_t* begin_ptr = nullptr;  // device begin/count (null: host values)
+    const int32_t* count_ptr = nullptr;
+    int begin = 0, count = 0;
+    int G = 8;                           // lanes per alignment
+    int bin = 0;                         // bin id (rescue list slot)
+    int max_blocks = 0;                  // grid cap (device count: the grid)
+};
+hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaunch& tl,
+                           hipStream_t st);
+hipError_t launch_fwd_tile_f64(int NR, const FwdParams& p, const TileLaunch& tl, hipStream_t st);
 hipError_t launch_fwd_wave_f32(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
                                const int32_t* count_ptr, const int32_t* hmax_ptr, float* spill,
                                int64_t spill_elems, int32_t* error_flag, int blocks,

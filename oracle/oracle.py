@@ -77,6 +77,12 @@ def ref_lib():
         _ref.ref_align_host.argtypes = [u8p, ctypes.c_int, u8p, ctypes.c_int, u8p, u8p, u8p,
                                         ctypes.c_uint8]
         _ref.ref_ph2pr.argtypes = [ctypes.POINTER(ctypes.c_float)]
+        ip = ctypes.POINTER(ctypes.c_int)
+        _ref.ref_score_batch.restype = ctypes.c_int
+        _ref.ref_score_batch.argtypes = [ctypes.c_int, ip, ip, ip, ip, u8p, u8p, u8p, u8p, u8p,
+                                         u8p, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.c_int64, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double)]
     return _ref
 
 
@@ -117,6 +123,28 @@ def ref_align(hap, read, base_q, ins_q, del_q, gcp0: int = 10) -> float:
     return rl.ref_align_host(_ptr(h, ctypes.c_uint8), len(h), _ptr(r, ctypes.c_uint8), len(r),
                              _ptr(bq, ctypes.c_uint8), _ptr(iq, ctypes.c_uint8),
                              _ptr(dq, ctypes.c_uint8), int(gcp0))
+
+
+def ref_score_batch(b, which=None, nthreads: int = 0) -> np.ndarray:
+    """The reference's align_host over a batch (or the listed alignments),
+    OpenMP over alignments like processBatchCPU."""
+    rl = ref_lib()
+    if rl is None:
+        raise RuntimeError("oracle/_ref not built")
+    args, keep = _batch_args(b)
+    if which is None:
+        n = int(np.sum(keep["nR"].astype(np.int64) * keep["nH"]))
+        wp = None
+    else:
+        w = np.ascontiguousarray(which, dtype=np.int64)
+        keep["which"] = w
+        n = len(w)
+        wp = _ptr(w, ctypes.c_int64)
+    out = np.empty(n, dtype=np.float64)
+    rc = rl.ref_score_batch(*args, wp, n, nthreads, _ptr(out, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("ref_score_batch failed")
+    return out
 
 
 def _batch_args(b):

@@ -66,17 +66,31 @@ static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
         Ip[c] = 0.0;
         Dp[c] = constant / H;
     }
+    /* Power-of-two row scaling keeps FP64 out of underflow for scores far
+     * below its range (log10 < -300): a row whose largest value drops under
+     * 2^-600 is multiplied by 2^600 (exact) and the exponent is carried. */
+    long exp2 = 0;
     for (int r = 1; r <= R; ++r) {
         const double p = P[qclamp(bq[r - 1])];
         const double delta = P[qclamp(iq[r - 1])];
         const double sigma = P[qclamp(dq[r - 1])];
         const double alpha = 1.0 - (delta + sigma);
         Mc[0] = Ic[0] = Dc[0] = 0.0;
+        double mx = 0.0;
         for (int c = 1; c <= H; ++c) {
             const double lambda = is_match(read[r - 1], hap[c - 1], semantics) ? 1.0 - p : p / 3.0;
             Mc[c] = lambda * (alpha * Mp[c - 1] + beta * (Ip[c - 1] + Dp[c - 1]));
             Ic[c] = delta * Mp[c] + eps * Ip[c];
             Dc[c] = sigma * Mc[c - 1] + eps * Dc[c - 1];
+            mx = fmax(mx, fmax(Mc[c], fmax(Ic[c], Dc[c])));
+        }
+        if (mx > 0.0 && mx < 0x1p-600) {
+            for (int c = 1; c <= H; ++c) {
+                Mc[c] = ldexp(Mc[c], 600);
+                Ic[c] = ldexp(Ic[c], 600);
+                Dc[c] = ldexp(Dc[c], 600);
+            }
+            exp2 -= 600;
         }
         double* t;
         t = Mp; Mp = Mc; Mc = t;
@@ -87,7 +101,7 @@ static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
     for (int c = 1; c <= H; ++c)
         sum += Mp[c] + Ip[c];
     free(buf);
-    return log10(sum) - log10(constant);
+    return log10(sum) + (double)exp2 * 0.30102999566398119521 - log10(constant);
 }
 
 /* Float cells, double result: the rounding of align_host, step for step. */
