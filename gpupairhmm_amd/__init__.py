@@ -101,7 +101,8 @@ class Stats(ctypes.Structure):
 
 
 class TileConfig(ctypes.Structure):
-    _fields_ = [("lanes", ctypes.c_int32), ("rows", ctypes.c_int32)]
+    _fields_ = [("lanes", ctypes.c_int32), ("rows", ctypes.c_int32),
+                ("max_read_len", ctypes.c_int32)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -567,14 +568,15 @@ class Engine:
     def reset_stats(self) -> None:
         self._chk(load_library().phmm_engine_reset_stats(self._h))
 
-    def tiles(self) -> list[tuple[int, int]]:
+    def tiles(self) -> list[tuple[int, int, int]]:
+        """The tile table: (lanes, rows, max_read_len) rows."""
         buf = (TileConfig * 64)()
         n = ctypes.c_int()
         self._chk(load_library().phmm_engine_get_tiles(self._h, buf, 64, ctypes.byref(n)))
-        return [(buf[i].lanes, buf[i].rows) for i in range(n.value)]
+        return [(buf[i].lanes, buf[i].rows, buf[i].max_read_len) for i in range(n.value)]
 
     def set_tiles(self, tiles) -> None:
-        arr = (TileConfig * len(tiles))(*[TileConfig(int(a), int(b)) for a, b in tiles])
+        arr = (TileConfig * len(tiles))(*[TileConfig(int(a), int(b), int(c)) for a, b, c in tiles])
         self._chk(load_library().phmm_engine_set_tiles(self._h, arr, len(tiles)))
 
 

@@ -57,9 +57,9 @@ __global__ void k_hap_max(BinParams bp)
         atomicMax(bp.hmax, m);
 }
 
-// Per read: its bin.  Tile class k is the smallest tile with room for the
-// read plus one out-of-bounds row; reads beyond the largest tile go to the
-// multi-tile bin 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when
+// Per read: its bin.  Tile class k is the first table row whose
+// max_read_len covers the read (its tile has room for one more row); reads
+// beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when
 // the read's coefficients keep it in range, else bin 2k+1 (SAFE).
 __global__ void k_read_meta(BinParams bp)
 {
@@ -76,7 +76,7 @@ __global__ void k_read_meta(BinParams bp)
     }
     int k = bp.ntiles;
     for (int j = 0; j < bp.ntiles; ++j)
-        if (L + 1 <= bp.tiles[j]) {
+        if (L <= bp.tiles[j]) {
             k = j;
             break;
         }

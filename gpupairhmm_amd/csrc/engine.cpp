@@ -21,12 +21,8 @@
 
 namespace phmm {
 
-// Default tile table: tiles (lanes x rows) ascending.  A read of length L
-// uses the smallest tile >= L + 1.
-const phmm_tile_config kDefaultTiles[16] = {
-    {2, 8},  {4, 8},  {6, 8},  {4, 16}, {5, 16}, {6, 16},  {7, 16},  {8, 16},
-    {8, 20}, {8, 24}, {8, 28}, {8, 32}, {10, 32}, {12, 32}, {16, 32}, {32, 32},
-};
+// Default tile table for gfx950, emitted by gpupairhmm_amd/autotune.py.
+#include "tuning_gfx950.inc"
 
 static thread_local std::string g_last_error;
 
@@ -245,7 +241,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
     for (int i = 0; i < ntiles; ++i)
-        bp.tiles[i] = tiles[(size_t)i].lanes * tiles[(size_t)i].rows;
+        bp.tiles[i] = tiles[(size_t)i].max_read_len;
     bp.semantics = opt.semantics;
     bp.allow_fast = 1;
     if ((rc = check(run_binning(bp, st), "binning kernels")))
@@ -357,7 +353,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
             if (opt.mode == PHMM_MODE_PLAIN && h_bin_range[2 * b + 1] <= 0)
                 continue;
             const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            const int tile = tc.lanes * tc.rows;
+            const int tile = tc.max_read_len + 1;  // rows the FP64 tile must hold
             if (tile <= 512) {
                 const int nr64 = tile <= 64 ? 4 : 8;
                 TileLaunch tl;
@@ -660,7 +656,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
     Engine& e = h->e;
     e.device = device;
     e.opt = o;
-    e.tiles.assign(std::begin(phmm::kDefaultTiles), std::end(phmm::kDefaultTiles));
+    e.tiles.assign(phmm::kDefaultTiles, phmm::kDefaultTiles + phmm::kNumDefaultTiles);
     int rc;
     auto bail = [&](int code) {
         const std::string m = e.err.empty() ? phmm::last_error() : e.err;
@@ -745,10 +741,11 @@ extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, 
     for (int i = 0; i < n; ++i) {
         const int tile = t[i].lanes * t[i].rows;
         if (t[i].lanes < 1 || t[i].lanes > 64 || t[i].rows < 4 || t[i].rows > 32 ||
-            t[i].rows % 4 != 0 || tile <= prev)
+            t[i].rows % 4 != 0 || t[i].max_read_len <= prev || tile < t[i].max_read_len + 1)
             return phmm::fail(PHMM_EINVAL,
-                              "phmm_engine_set_tiles: lanes 1..64, rows 4..32 step 4, tiles ascending");
-        prev = tile;
+                              "phmm_engine_set_tiles: lanes 1..64, rows 4..32 step 4, max_read_len "
+                              "ascending and <= lanes*rows - 1");
+        prev = t[i].max_read_len;
     }
     std::lock_guard<std::mutex> lk(h->mu);
     h->e.tiles.assign(t, t + n);

@@ -85,7 +85,8 @@ struct Engine {
     void free_slot(Slot& s);
 };
 
-extern const phmm_tile_config kDefaultTiles[16];
+extern const phmm_tile_config kDefaultTiles[];
+extern const int kNumDefaultTiles;
 
 }  // namespace phmm
 

@@ -261,18 +261,43 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
     const int hdw_n = active ? (HL >> 2) + 1 : 0;  // padded length / 4
     uint32_t wcur = (first && 0 < hdw_n) ? hdw[0] : 0x04040404u;
     uint32_t wnext = (first && 1 < hdw_n) ? hdw[1] : 0x04040404u;
+    // the chain is exactly 0 before a lane's first column, so only the end
+    // of the lane's real columns needs a bound: step s adds column s - t,
+    // which is real while s <= t + HL
+    const int acc_last = t + HL;
+    const int lane_off = lane;  // tab index of this lane's letter-0 entry
+
+    // letter and emission values of step 0; each step loads the next step's
+    // values before it computes (the next letter is known one step ahead)
+    letter = first ? (int)(wcur & 0xffu) : 4;
+    V lamv[NQ];
+    {
+        const V* trow = tab + __umul24((unsigned)letter, NQ * 64) + lane_off;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            lamv[q] = trow[q * 64];
+    }
 
     for (int s4 = 0; s4 < nsteps; s4 += 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
+            // next step's letter: the group's first lane reads the hap, the
+            // others take their left neighbour's current letter
             const int sh = shr1i(letter);
-            letter = first ? (int)((wcur >> (8 * u)) & 0xffu) : sh;
-            const V* trow = tab + letter * (NQ * 64) + lane;
+            const uint32_t wsrc = (u == 3) ? wnext : wcur;
+            const int nletter = first ? (int)((wsrc >> (8 * ((u + 1) & 3))) & 0xffu) : sh;
+            V lamn[NQ];
+            {
+                const V* trow = tab + __umul24((unsigned)nletter, NQ * 64) + lane_off;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    lamn[q] = trow[q * 64];
+            }
             T mp = Mul, dp = Dul, chain = Iin;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 T lam[K];
-                TT::unpack(trow[q * 64], lam);
+                TT::unpack(lamv[q], lam);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int j = K * q + k;
@@ -293,8 +318,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                     dp = dov;
                 }
             }
-            const int c = s4 + u - t;
-            acc += (c >= 1 && c <= HL) ? chain : (T)0;
+            acc += (s4 + u <= acc_last) ? chain : (T)0;
             const T sM = TT::shr(M[NR - 1]);
             const T sD = TT::shr(D[NR - 1]);
             const T sI = TT::shr(chain);
@@ -303,6 +327,10 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
             Ml = first ? (T)0 : sM;
             Dl = first ? C0 : sD;
             Iin = first ? (T)0 : sI;
+            letter = nletter;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                lamv[q] = lamn[q];
         }
         const int k = (s4 >> 2) + 2;
         wcur = wnext;

@@ -130,7 +130,7 @@ struct BinParams {
     const float* ph2pr;
     // tiles
     int32_t ntiles;
-    int32_t tiles[kMaxTiles];  // ascending tile sizes (lanes*rows)
+    int32_t tiles[kMaxTiles];  // ascending max read length per tile class
     int32_t semantics;
     int32_t allow_fast;
 };

@@ -127,11 +127,14 @@ typedef struct phmm_stats {
     int64_t reserved[8];
 } phmm_stats;
 
-/* One row of the tile table: reads with readLen+1 <= tile use `lanes`
- * lanes x `rows` rows per lane (tile = lanes*rows). */
+/* One row of the tile table (the reference's per-architecture
+ * tuningconfigs.hpp:33-239, here chosen on gfx950 by the autotuner): reads
+ * of length <= max_read_len (and above the previous row's) use `lanes`
+ * lanes x `rows` rows per lane; lanes*rows must be >= max_read_len + 1. */
 typedef struct phmm_tile_config {
-    int32_t lanes;   /* lanes per alignment, 1..64 */
-    int32_t rows;    /* rows per lane: 4, 8, 12, 16, 20, 24, 28 or 32 */
+    int32_t lanes;         /* lanes per alignment, 1..64 */
+    int32_t rows;          /* rows per lane: 4, 8, 12, 16, 20, 24, 28 or 32 */
+    int32_t max_read_len;  /* ascending across the table */
 } phmm_tile_config;
 
 /* ---- library --------------------------------------------------------- */
@@ -145,7 +148,8 @@ void phmm_engine_destroy(phmm_engine* eng);
 const char* phmm_engine_last_error(const phmm_engine* eng);
 int phmm_engine_stats(const phmm_engine* eng, phmm_stats* out);
 int phmm_engine_reset_stats(phmm_engine* eng);
-/* Replace the tile table (ascending tiles); n <= 32. */
+/* Replace the tile table (max_read_len ascending); n <= 32.  Reads longer
+ * than the last row's max_read_len use the multi-tile kernel. */
 int phmm_engine_set_tiles(phmm_engine* eng, const phmm_tile_config* tiles, int n);
 int phmm_engine_get_tiles(const phmm_engine* eng, phmm_tile_config* tiles, int cap, int* n);
 

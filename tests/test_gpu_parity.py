@@ -94,8 +94,8 @@ def test_edge_cases(engine, oracle):
     rs = lambda n: "".join(rng.choice(list("ACGT"), n)) if n else ""
     recs = []
     # tile boundaries of the default table and reads around them
-    tiles = [a * b for a, b in engine.tiles()]
-    lens = sorted({1, 2, 3, 5, 17} | {t + d for t in tiles for d in (-2, -1, 0) if t + d > 0})
+    bounds = [m for _, _, m in engine.tiles()]
+    lens = sorted({1, 2, 3, 5, 17} | {t + d for t in bounds for d in (-1, 0, 1) if t + d > 0})
     for L in lens:
         recs.append(_record([rs(L)], [rs(max(L, 40)), rs(7)]))
     # wildcards and non-ACGT bytes on both sides, hap of length 1
