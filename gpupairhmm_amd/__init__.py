@@ -73,7 +73,8 @@ class Options(ctypes.Structure):
         ("transfer_batchsize", ctypes.c_int64), ("mode", ctypes.c_int32),
         ("semantics", ctypes.c_int32), ("fp64_rescue", ctypes.c_int32),
         ("rescue_min_sum", ctypes.c_float), ("verbose", ctypes.c_int32),
-        ("time_kernels", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("time_kernels", ctypes.c_int32), ("graphs", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -96,7 +97,8 @@ class Stats(ctypes.Structure):
         ("alignments", ctypes.c_int64), ("cells", ctypes.c_int64), ("rescued", ctypes.c_int64),
         ("fast_alignments", ctypes.c_int64), ("multitile_alignments", ctypes.c_int64),
         ("engine_seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
-        ("reserved", ctypes.c_int64 * 8),
+        ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 6),
     ]
 
 
@@ -449,6 +451,8 @@ class EngineStats:
     multitile_alignments: int
     engine_seconds: float
     kernel_seconds: float
+    graph_launches: int = 0
+    graph_captures: int = 0
 
 
 class DeviceBatch:
@@ -480,7 +484,8 @@ class Engine:
 
     def __init__(self, device: int = 0, mode: int = MODE_BATCHED, semantics: int = SEM_GPU,
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
-                 rescue_min_sum: float = 1e-28, time_kernels: bool = False):
+                 rescue_min_sum: float = 1e-28, time_kernels: bool = False,
+                 graphs: bool = True):
         lib = load_library()
         o = Options()
         lib.phmm_options_default(ctypes.byref(o))
@@ -490,6 +495,7 @@ class Engine:
         o.fp64_rescue = 1 if fp64_rescue else 0
         o.rescue_min_sum = float(rescue_min_sum)
         o.time_kernels = 1 if time_kernels else 0
+        o.graphs = 1 if graphs else 0
         h = ctypes.c_void_p()
         _check(lib.phmm_engine_create(int(device), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -563,7 +569,8 @@ class Engine:
         s = Stats()
         self._chk(load_library().phmm_engine_stats(self._h, ctypes.byref(s)))
         return EngineStats(s.alignments, s.cells, s.rescued, s.fast_alignments,
-                           s.multitile_alignments, s.engine_seconds, s.kernel_seconds)
+                           s.multitile_alignments, s.engine_seconds, s.kernel_seconds,
+                           s.graph_launches, s.graph_captures)
 
     def reset_stats(self) -> None:
         self._chk(load_library().phmm_engine_reset_stats(self._h))

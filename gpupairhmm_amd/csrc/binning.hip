@@ -27,30 +27,38 @@ __global__ void k_clear(int32_t* p, int64_t n)
         p[i] = 0;
 }
 
+// Grid-stride loops throughout: a launch covers any batch size, so the
+// same grids serve every call when the sequence is captured in a graph.
+#define GRID_STRIDE(i, n)                                                              \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)(n); \
+         i += (int64_t)gridDim.x * blockDim.x)
+
 __global__ void k_encode_reads(BinParams bp)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bp.read_bytes;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = (uint32_t)base_code(bp.read_data[i]);
-        const uint32_t q = min((uint32_t)bp.bq[i], 127u);
-        const uint32_t qi = min((uint32_t)bp.iq[i], 127u);
-        const uint32_t qd = min((uint32_t)bp.dq[i], 127u);
-        bp.rows[i] = b | (q << 8) | (qi << 16) | (qd << 24);
+    const BatchDesc& b = *bp.bd;
+    GRID_STRIDE(i, b.read_bytes)
+    {
+        const uint32_t c = (uint32_t)base_code(b.read_data[i]);
+        const uint32_t q = min((uint32_t)b.bq[i], 127u);
+        const uint32_t qi = min((uint32_t)b.iq[i], 127u);
+        const uint32_t qd = min((uint32_t)b.dq[i], 127u);
+        bp.rows[i] = c | (q << 8) | (qi << 16) | (qd << 24);
     }
 }
 
 __global__ void k_encode_haps(BinParams bp)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bp.hap_bytes;
-         i += (int64_t)gridDim.x * blockDim.x)
-        bp.hapc[i] = (uint8_t)base_code(bp.hap_data[i]);
+    const BatchDesc& b = *bp.bd;
+    GRID_STRIDE(i, b.hap_bytes)
+    bp.hapc[i] = (uint8_t)base_code(b.hap_data[i]);
 }
 
 __global__ void k_hap_max(BinParams bp)
 {
+    const BatchDesc& b = *bp.bd;
     int m = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < bp.nH; i += gridDim.x * blockDim.x)
-        m = max(m, bp.hap_len[i]);
+    GRID_STRIDE(i, b.nH)
+    m = max(m, b.hap_len[i]);
     for (int o = 32; o > 0; o >>= 1)
         m = max(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0)
@@ -59,72 +67,76 @@ __global__ void k_hap_max(BinParams bp)
 
 // Per read: its bin.  Tile class k is the first table row whose
 // max_read_len covers the read (its tile has room for one more row); reads
-// beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when
-// the read's coefficients keep it in range, else bin 2k+1 (SAFE).
+// beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses the
+// rescaled (FAST) arithmetic when the read's coefficients keep it in range,
+// else bin 2k+1 (SAFE).
 __global__ void k_read_meta(BinParams bp)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= bp.nR)
-        return;
-    const int L = bp.read_len[i];
-    ReadMeta m;
-    m.nl = 1.0f;
-    if (L <= 0) {
-        m.bin = -1;
-        bp.meta[i] = m;
-        return;
-    }
-    int k = bp.ntiles;
-    for (int j = 0; j < bp.ntiles; ++j)
-        if (L <= bp.tiles[j]) {
-            k = j;
-            break;
+    const BatchDesc& b = *bp.bd;
+    GRID_STRIDE(i, b.nR)
+    {
+        const int L = b.read_len[i];
+        ReadMeta m;
+        m.nl = 1.0f;
+        if (L <= 0) {
+            m.bin = -1;
+            bp.meta[i] = m;
+            continue;
         }
-    if (k == bp.ntiles) {
-        m.bin = 2 * bp.ntiles;
+        int k = bp.ntiles;
+        for (int j = 0; j < bp.ntiles; ++j)
+            if (L <= bp.tiles[j]) {
+                k = j;
+                break;
+            }
+        if (k == bp.ntiles) {
+            m.bin = 2 * bp.ntiles;
+            bp.meta[i] = m;
+            continue;
+        }
+        const int off = b.read_off[i];
+        float eps = kEpsGpu;
+        if (bp.semantics == 1)
+            eps = bp.ph2pr[min((int)b.gcp[off], 127)];
+        bool ok = bp.allow_fast != 0 && eps <= 0.5f;
+        double nl = 1.0;
+        float pdelta = 0.f, palpha = 1.f;
+        for (int r = 0; r < L && ok; ++r) {
+            const uint32_t w = bp.rows[off + r];
+            const float delta = bp.ph2pr[row_iq(w)];
+            const float sigma = bp.ph2pr[row_dq(w)];
+            const float alpha = 1.0f - (delta + sigma);
+            if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
+                ok = false;
+            else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
+                ok = false;
+            nl *= (double)alpha;
+            pdelta = delta;
+            palpha = alpha;
+        }
+        if (ok && !(nl >= 1e-30))  // the start constant must stay a normal float
+            ok = false;
+        m.nl = ok ? (float)nl : 1.0f;
+        m.bin = 2 * k + (ok ? 0 : 1);
         bp.meta[i] = m;
-        return;
     }
-    const int off = bp.read_off[i];
-    float eps = kEpsGpu;
-    if (bp.semantics == 1)
-        eps = bp.ph2pr[min((int)bp.gcp[off], 127)];
-    bool ok = bp.allow_fast != 0 && eps <= 0.5f;
-    double nl = 1.0;
-    float pdelta = 0.f, palpha = 1.f;
-    for (int r = 0; r < L && ok; ++r) {
-        const uint32_t w = bp.rows[off + r];
-        const float delta = bp.ph2pr[row_iq(w)];
-        const float sigma = bp.ph2pr[row_dq(w)];
-        const float alpha = 1.0f - (delta + sigma);
-        if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
-            ok = false;
-        else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
-            ok = false;
-        nl *= (double)alpha;
-        pdelta = delta;
-        palpha = alpha;
-    }
-    if (ok && !(nl >= 1e-30))  // the start constant must stay a normal float
-        ok = false;
-    m.nl = ok ? (float)nl : 1.0f;
-    m.bin = 2 * k + (ok ? 0 : 1);
-    bp.meta[i] = m;
 }
 
 // Exclusive prefix of R_g*H_g (the reference's resultOffsetsPerBatch,
 // gpuworker.cuh:802-815), one block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
 {
+    const BatchDesc& b = *bp.bd;
+    const int G = b.G;
     __shared__ int64_t wsum[16];
     __shared__ int64_t carry;
     if (threadIdx.x == 0)
         carry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int base = 0; base < bp.G; base += 1024) {
+    for (int base = 0; base < G; base += 1024) {
         const int g = base + threadIdx.x;
-        int64_t v = g < bp.G ? (int64_t)bp.nreads[g] * bp.nhaps[g] : 0;
+        int64_t v = g < G ? (int64_t)b.nreads[g] * b.nhaps[g] : 0;
         int64_t x = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int64_t y = __shfl_up(x, o, 64);
@@ -144,7 +156,7 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         }
         __syncthreads();
         const int64_t incl = x + wsum[wv] + carry;
-        if (g < bp.G)
+        if (g < G)
             bp.align_off[g] = incl - v;
         __syncthreads();
         if (threadIdx.x == 1023)
@@ -152,39 +164,43 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        bp.align_off[bp.G] = carry;
+        bp.align_off[G] = carry;
 }
 
-// One block per group: count its reads per bin, lay them out bin by bin.
+// Per group (grid-stride over groups, one block each): count its reads per
+// bin, lay them out bin by bin.
 __global__ __launch_bounds__(256) void k_group_bins(BinParams bp)
 {
+    const BatchDesc& b = *bp.bd;
     __shared__ int cnt[2 * kMaxTiles + 2];
     __shared__ int pos[2 * kMaxTiles + 2];
-    const int g = blockIdx.x;
     const int NB1 = bp.NB + 1;  // slot NB: empty reads
-    for (int b = threadIdx.x; b < NB1; b += blockDim.x)
-        cnt[b] = 0;
-    __syncthreads();
-    const int r0 = bp.rprefix[g], nr = bp.nreads[g];
-    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-        const int b = bp.meta[r0 + r].bin;
-        atomicAdd(&cnt[b < 0 ? bp.NB : b], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int b = 0; b < NB1; ++b) {
-            pos[b] = s;
-            bp.gb_start[(int64_t)g * NB1 + b] = s;
-            bp.gb_count[(int64_t)g * NB1 + b] = cnt[b];
-            s += cnt[b];
+    for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        for (int k = threadIdx.x; k < NB1; k += blockDim.x)
+            cnt[k] = 0;
+        __syncthreads();
+        const int r0 = b.rprefix[g], nr = b.nreads[g];
+        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+            const int k = bp.meta[r0 + r].bin;
+            atomicAdd(&cnt[k < 0 ? bp.NB : k], 1);
         }
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-        const int b = bp.meta[r0 + r].bin;
-        const int k = atomicAdd(&pos[b < 0 ? bp.NB : b], 1);
-        bp.read_perm[r0 + k] = r0 + r;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int k = 0; k < NB1; ++k) {
+                pos[k] = s;
+                bp.gb_start[(int64_t)g * NB1 + k] = s;
+                bp.gb_count[(int64_t)g * NB1 + k] = cnt[k];
+                s += cnt[k];
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+            const int k = bp.meta[r0 + r].bin;
+            const int at = atomicAdd(&pos[k < 0 ? bp.NB : k], 1);
+            bp.read_perm[r0 + at] = r0 + r;
+        }
+        __syncthreads();
     }
 }
 
@@ -193,21 +209,22 @@ __device__ __forceinline__ int key_of(int bin, int hl)
     return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
 }
 
-// One block per group: add its alignment counts to the (bin, hap length)
-// histogram.
+// Per group: add its alignment counts to the (bin, hap length) histogram.
 __global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
 {
-    const int g = blockIdx.x;
+    const BatchDesc& b = *bp.bd;
     const int NB1 = bp.NB + 1;
-    const int h0 = bp.hprefix[g], nh = bp.nhaps[g];
-    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-        const int hl = bp.hap_len[h0 + h];
-        if (hl <= 0)
-            continue;
-        for (int b = 0; b < bp.NB; ++b) {
-            const int n = bp.gb_count[(int64_t)g * NB1 + b];
-            if (n > 0)
-                atomicAdd(&bp.key_count[key_of(b, hl)], n);
+    for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        const int h0 = b.hprefix[g], nh = b.nhaps[g];
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const int hl = b.hap_len[h0 + h];
+            if (hl <= 0)
+                continue;
+            for (int k = 0; k < bp.NB; ++k) {
+                const int n = bp.gb_count[(int64_t)g * NB1 + k];
+                if (n > 0)
+                    atomicAdd(&bp.key_count[key_of(k, hl)], n);
+            }
         }
     }
 }
@@ -255,50 +272,52 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
         __syncthreads();
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < bp.NB; b += blockDim.x) {
-        const int end = (b + 1 < bp.NB) ? bp.bin_range[2 * (b + 1)] : carry;
-        bp.bin_range[2 * b + 1] = end - bp.bin_range[2 * b];
+    for (int k = threadIdx.x; k < bp.NB; k += blockDim.x) {
+        const int end = (k + 1 < bp.NB) ? bp.bin_range[2 * (k + 1)] : carry;
+        bp.bin_range[2 * k + 1] = end - bp.bin_range[2 * k];
     }
 }
 
-// One block per group: reserve each (hap, bin) run and write its items;
-// alignments with an empty read or hap get -inf directly (align_host returns
+// Per group: reserve each (hap, bin) run and write its items; alignments
+// with an empty read or hap get -inf directly (align_host returns
 // log10(0) - log10(c) for them).
 __global__ __launch_bounds__(256) void k_scatter(BinParams bp)
 {
-    const int g = blockIdx.x;
+    const BatchDesc& b = *bp.bd;
     const int NB1 = bp.NB + 1;
-    const int r0 = bp.rprefix[g], nr = bp.nreads[g];
-    const int h0 = bp.hprefix[g], nh = bp.nhaps[g];
-    const int64_t A = bp.align_off[g];
-    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-        const int hl = bp.hap_len[h0 + h];
-        if (hl <= 0) {
-            for (int r = 0; r < nr; ++r)
-                bp.scores[A + (int64_t)r * nh + h] = -INFINITY;
-            continue;
-        }
-        for (int b = 0; b < NB1; ++b) {
-            const int n = bp.gb_count[(int64_t)g * NB1 + b];
-            if (n <= 0)
-                continue;
-            const int st = bp.gb_start[(int64_t)g * NB1 + b];
-            if (b == bp.NB) {
-                for (int k = 0; k < n; ++k) {
-                    const int rd = bp.read_perm[r0 + st + k];
-                    bp.scores[A + (int64_t)(rd - r0) * nh + h] = -INFINITY;
-                }
+    for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        const int r0 = b.rprefix[g], nr = b.nreads[g];
+        const int h0 = b.hprefix[g], nh = b.nhaps[g];
+        const int64_t A = bp.align_off[g];
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const int hl = b.hap_len[h0 + h];
+            if (hl <= 0) {
+                for (int r = 0; r < nr; ++r)
+                    b.scores[A + (int64_t)r * nh + h] = -INFINITY;
                 continue;
             }
-            const int base = atomicAdd(&bp.key_cursor[key_of(b, hl)], n);
-            for (int k = 0; k < n; ++k) {
-                const int rd = bp.read_perm[r0 + st + k];
-                WorkItem it;
-                it.read = rd;
-                it.hap = h0 + h;
-                it.out = (int32_t)(A + (int64_t)(rd - r0) * nh + h);
-                it.pad = 0;
-                bp.items[base + k] = it;
+            for (int k = 0; k < NB1; ++k) {
+                const int n = bp.gb_count[(int64_t)g * NB1 + k];
+                if (n <= 0)
+                    continue;
+                const int st = bp.gb_start[(int64_t)g * NB1 + k];
+                if (k == bp.NB) {
+                    for (int j = 0; j < n; ++j) {
+                        const int rd = bp.read_perm[r0 + st + j];
+                        b.scores[A + (int64_t)(rd - r0) * nh + h] = -INFINITY;
+                    }
+                    continue;
+                }
+                const int base = atomicAdd(&bp.key_cursor[key_of(k, hl)], n);
+                for (int j = 0; j < n; ++j) {
+                    const int rd = bp.read_perm[r0 + st + j];
+                    WorkItem it;
+                    it.read = rd;
+                    it.hap = h0 + h;
+                    it.out = (int32_t)(A + (int64_t)(rd - r0) * nh + h);
+                    it.pad = 0;
+                    bp.items[base + j] = it;
+                }
             }
         }
     }
@@ -322,36 +341,39 @@ hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStr
 }
 
 // ---------------------------------------------------------------------------
-hipError_t run_binning(const BinParams& bp, hipStream_t st)
+hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, hipStream_t st)
 {
     const int tpb = 256;
-    auto blocks_for = [](int64_t n, int tpb_) {
-        int64_t b = (n + tpb_ - 1) / tpb_;
-        return (int)(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+    // grids: sized to the batch, or fixed (graph capture) at a size that
+    // still fills the chip
+    auto grid = [&](int64_t n, int per_block, int cap) {
+        if (fixed_grids)
+            return cap;
+        int64_t b = (n + per_block - 1) / per_block;
+        return (int)(b < cap ? (b < 1 ? 1 : b) : cap);
     };
-    const int NB1 = bp.NB + 1;
-    hipLaunchKernelGGL(k_clear, dim3(blocks_for((int64_t)bp.NB * kHapBuckets, tpb)), dim3(tpb), 0,
+    hipLaunchKernelGGL(k_clear, dim3(grid((int64_t)bp.NB * kHapBuckets, tpb, 512)), dim3(tpb), 0,
                        st, bp.key_count, (int64_t)bp.NB * kHapBuckets);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.hmax, (int64_t)1);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
-    if (bp.read_bytes > 0)
-        hipLaunchKernelGGL(k_encode_reads, dim3(blocks_for(bp.read_bytes, tpb)), dim3(tpb), 0, st, bp);
-    if (bp.hap_bytes > 0)
-        hipLaunchKernelGGL(k_encode_haps, dim3(blocks_for(bp.hap_bytes, tpb)), dim3(tpb), 0, st, bp);
-    if (bp.nH > 0)
-        hipLaunchKernelGGL(k_hap_max, dim3(blocks_for(bp.nH, tpb)), dim3(tpb), 0, st, bp);
-    if (bp.nR > 0)
-        hipLaunchKernelGGL(k_read_meta, dim3((bp.nR + tpb - 1) / tpb), dim3(tpb), 0, st, bp);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.task_ctr, (int64_t)kTaskCounters);
+    if (fixed_grids || hd.read_bytes > 0)
+        hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes, tpb, 4096)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.hap_bytes > 0)
+        hipLaunchKernelGGL(k_encode_haps, dim3(grid(hd.hap_bytes, tpb, 2048)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.nH > 0)
+        hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.nR > 0)
+        hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb, 2048)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
-    if (bp.G > 0) {
-        hipLaunchKernelGGL(k_group_bins, dim3(bp.G), dim3(256), 0, st, bp);
-        hipLaunchKernelGGL(k_count_keys, dim3(bp.G), dim3(256), 0, st, bp);
+    if (fixed_grids || hd.G > 0) {
+        hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
+        hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
     }
     hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, bp);
-    if (bp.G > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(bp.G), dim3(256), 0, st, bp);
-    (void)NB1;
+    if (fixed_grids || hd.G > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
     return hipGetLastError();
 }
 

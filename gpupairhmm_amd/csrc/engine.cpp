@@ -65,6 +65,18 @@ Engine::~Engine()
             hipFree(b->p);
     for (auto& s : slots)
         free_slot(s);
+    drop_graphs();
+    if (desc.p)
+        hipFree(desc.p);
+    if (h_desc)
+        hipHostFree(h_desc);
+    for (auto& ev : desc_ev)
+        if (ev)
+            hipEventDestroy(ev);
+    if (call_done)
+        hipEventDestroy(call_done);
+    if (cap_stream)
+        hipStreamDestroy(cap_stream);
     if (d_rescued_total)
         hipFree(d_rescued_total);
     if (h_bin_range)
@@ -88,13 +100,15 @@ int Engine::ensure(DevBuf& b, size_t bytes, const char* what)
 {
     if (bytes <= b.cap && b.p)
         return PHMM_OK;
+    size_t nb = std::max<size_t>(bytes, 256);
     if (b.p) {
         hipDeviceSynchronize();  // in-flight kernels may still use it
         hipFree(b.p);
+        nb = std::max(nb, b.cap + b.cap / 2);  // grow geometrically
         b.p = nullptr;
         b.cap = 0;
     }
-    size_t nb = std::max<size_t>(bytes, 256);
+    ++generation;
     nb = (nb + 255) & ~(size_t)255;
     if (hipMalloc(&b.p, nb) != hipSuccess) {
         hipGetLastError();
@@ -187,41 +201,70 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
         (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")) ||
-        (rc = ensure(final_list, sizeof(int32_t) * (size_t)nA, "final list")))
+        (rc = ensure(final_list, sizeof(int32_t) * (size_t)nA, "final list")) ||
+        (rc = ensure(desc, sizeof(BatchDesc), "batch descriptor")))
         return rc;
     const size_t spill_bytes = (size_t)256 << 20;
     if ((rc = ensure(spill, spill_bytes, "spill")))
         return rc;
+    if (!h_desc) {
+        if ((rc = check(hipHostMalloc(reinterpret_cast<void**>(&h_desc),
+                                      sizeof(BatchDesc) * kDescRing, hipHostMallocDefault),
+                        "pinned descriptors")))
+            return rc;
+        for (int k = 0; k < kDescRing; ++k)
+            if ((rc = check(hipEventCreateWithFlags(&desc_ev[k], hipEventDisableTiming), "event")))
+                return rc;
+        if ((rc = check(hipEventCreateWithFlags(&call_done, hipEventDisableTiming), "event")))
+            return rc;
+    }
+    if (call_recorded && last_stream != st &&
+        (rc = check(hipStreamWaitEvent(st, call_done, 0), "event wait")))
+        return rc;
+
+    BatchDesc hd{};
+    hd.read_data = v.read_data;
+    hd.hap_data = v.hap_data;
+    hd.bq = v.base_quals;
+    hd.iq = v.ins_quals;
+    hd.dq = v.del_quals;
+    hd.gcp = v.gcp_quals;
+    hd.read_len = v.read_lengths;
+    hd.hap_len = v.hap_lengths;
+    hd.read_off = v.read_offsets;
+    hd.hap_off = v.hap_offsets;
+    hd.nreads = v.num_reads_per_group;
+    hd.nhaps = v.num_haps_per_group;
+    hd.rprefix = v.reads_prefix;
+    hd.hprefix = v.haps_prefix;
+    hd.scores = d_scores;
+    hd.G = G;
+    hd.nR = v.num_reads;
+    hd.nH = v.num_haps;
+    hd.read_bytes = v.read_bytes;
+    hd.hap_bytes = v.hap_bytes;
+    hd.num_alignments = nA;
+    {
+        const int k = desc_next;
+        desc_next = (desc_next + 1) % kDescRing;
+        if (desc_used[k] && (rc = check(hipEventSynchronize(desc_ev[k]), "descriptor slot")))
+            return rc;
+        h_desc[k] = hd;
+        if ((rc = check(hipMemcpyAsync(desc.p, &h_desc[k], sizeof(BatchDesc),
+                                       hipMemcpyHostToDevice, st),
+                        "descriptor copy")) ||
+            (rc = check(hipEventRecord(desc_ev[k], st), "event")))
+            return rc;
+        desc_used[k] = true;
+    }
 
     // counters: [0] max hap length, [1] final-list count, [2] error flag,
-    // [4 .. 4+NB) per-bin rescue counts
+    // [4 .. 4+NB) per-bin rescue counts, [128 .. 128+kTaskCounters) task
+    // counters of the set launches
     int32_t* sm = static_cast<int32_t*>(smallints.p);
-    int32_t* d_hmax = sm + 0;
-    int32_t* d_final_count = sm + 1;
-    int32_t* d_error = sm + 2;
-    int32_t* d_rescue_counts = sm + 4;
-
     BinParams bp{};
-    bp.read_data = v.read_data;
-    bp.hap_data = v.hap_data;
-    bp.bq = v.base_quals;
-    bp.iq = v.ins_quals;
-    bp.dq = v.del_quals;
-    bp.gcp = v.gcp_quals;
-    bp.read_len = v.read_lengths;
-    bp.hap_len = v.hap_lengths;
-    bp.read_off = v.read_offsets;
-    bp.hap_off = v.hap_offsets;
-    bp.nreads = v.num_reads_per_group;
-    bp.nhaps = v.num_haps_per_group;
-    bp.rprefix = v.reads_prefix;
-    bp.hprefix = v.haps_prefix;
-    bp.G = G;
-    bp.nR = v.num_reads;
-    bp.nH = v.num_haps;
+    bp.bd = static_cast<const BatchDesc*>(desc.p);
     bp.NB = NB;
-    bp.read_bytes = v.read_bytes;
-    bp.hap_bytes = v.hap_bytes;
     bp.rows = static_cast<uint32_t*>(rows.p);
     bp.hapc = static_cast<uint8_t*>(hapc.p);
     bp.meta = static_cast<ReadMeta*>(meta.p);
@@ -232,37 +275,30 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.key_count = static_cast<int32_t*>(key_count.p);
     bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
     bp.bin_range = static_cast<int32_t*>(bin_range.p);
-    bp.hmax = d_hmax;
-    bp.rescue_counts = d_rescue_counts;
-    bp.final_count = d_final_count;
-    bp.error_flag = d_error;
+    bp.hmax = sm + 0;
+    bp.final_count = sm + 1;
+    bp.error_flag = sm + 2;
+    bp.rescue_counts = sm + 4;
+    bp.task_ctr = sm + 128;
     bp.items = static_cast<WorkItem*>(items.p);
-    bp.scores = d_scores;
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
     for (int i = 0; i < ntiles; ++i)
         bp.tiles[i] = tiles[(size_t)i].max_read_len;
     bp.semantics = opt.semantics;
     bp.allow_fast = 1;
-    if ((rc = check(run_binning(bp, st), "binning kernels")))
-        return rc;
 
     FwdParams fp{};
+    fp.bd = bp.bd;
     fp.items = bp.items;
     fp.rows = bp.rows;
     fp.hapc = bp.hapc;
-    fp.gcp = v.gcp_quals;
-    fp.read_len = v.read_lengths;
-    fp.hap_len = v.hap_lengths;
-    fp.read_off = v.read_offsets;
-    fp.hap_off = v.hap_offsets;
     fp.meta = bp.meta;
     fp.ph2pr = d_ph2pr;
-    fp.scores = d_scores;
     fp.rescue_list = static_cast<int32_t*>(rescue_list.p);
-    fp.rescue_counts = d_rescue_counts;
+    fp.rescue_counts = bp.rescue_counts;
     fp.final_list = static_cast<int32_t*>(final_list.p);
-    fp.final_count = d_final_count;
+    fp.final_count = bp.final_count;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;
@@ -279,109 +315,245 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
             ev_free.pop_back();
         }
     }
-    const int persist_blocks = num_cus * 8;
-    const int mb = 2 * ntiles;  // multi-tile bin
-    float* spill32 = static_cast<float*>(spill.p);
-    double* spill64 = static_cast<double*>(spill.p);
-    const int64_t n32 = (int64_t)(spill.cap / sizeof(float)), n64 = (int64_t)(spill.cap / sizeof(double));
-    // pass 1: FP32 tile kernels per bin (results below the rescue threshold
-    // go to the bin's rescue list), FP32 multi-tile kernel for long reads
-    if (opt.mode == PHMM_MODE_PLAIN) {
-        // one host round trip per batch for the per-bin sizes (the reference
-        // syncs per chunk for its per-partition counts, gpuworker.cuh:852-854)
-        if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NB,
-                                       hipMemcpyDeviceToHost, st),
-                        "bin range readback")) ||
-            (rc = check(hipStreamSynchronize(st), "bin range sync")))
+    const bool plain = opt.mode == PHMM_MODE_PLAIN;
+    if (plain || !opt.graphs) {
+        if ((rc = enqueue_prep(bp, hd, false, st)))
             return rc;
+        if (plain) {
+            // one host round trip per batch for the per-bin sizes (the
+            // reference syncs per chunk for its per-partition counts,
+            // gpuworker.cuh:852-854)
+            if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NB,
+                                           hipMemcpyDeviceToHost, st),
+                            "bin range readback")) ||
+                (rc = check(hipStreamSynchronize(st), "bin range sync")))
+                return rc;
+        }
         if (timed)
             hipEventRecord(evp.first, st);
+        if ((rc = enqueue_fwd(fp, bp, plain, st)))
+            return rc;
+        if (timed)
+            hipEventRecord(evp.second, st);
+        if (opt.fp64_rescue && (rc = enqueue_rescue(fp, bp, plain, st)))
+            return rc;
+    } else {
+        // batched: replay captured graphs -- one for the whole call, or one
+        // per stage when the forward kernels are timed
+        if (graphs.gen != generation) {
+            drop_graphs();
+            graphs.gen = generation;
+        }
+        auto launch = [&](int stages, hipGraphExec_t* ex) -> int {
+            int r2;
+            if (!*ex && (r2 = capture(stages, bp, fp, hd, ex)))
+                return r2;
+            ++graph_launches;
+            return check(hipGraphLaunch(*ex, st), "graph launch");
+        };
+        if (!timed) {
+            if ((rc = launch(kStagePrep | kStageFwd | kStageRescue, &graphs.all)))
+                return rc;
+        } else {
+            if ((rc = launch(kStagePrep, &graphs.prep)))
+                return rc;
+            hipEventRecord(evp.first, st);
+            if ((rc = launch(kStageFwd, &graphs.fwd)))
+                return rc;
+            hipEventRecord(evp.second, st);
+            if (opt.fp64_rescue && (rc = launch(kStageRescue, &graphs.rescue)))
+                return rc;
+        }
+    }
+    if (timed)
+        ev_pending.push_back(evp);
+    if ((rc = check(hipEventRecord(call_done, st), "event")))
+        return rc;
+    last_stream = st;
+    call_recorded = true;
+    stats.alignments += nA;
+    return PHMM_OK;
+}
+
+int Engine::enqueue_prep(const BinParams& bp, const BatchDesc& hd, bool fixed, hipStream_t st)
+{
+    return check(run_binning(bp, hd, fixed, st), "binning kernels");
+}
+
+// Pass 1: FP32 tile kernels (results below the rescue threshold go to the
+// bin's rescue list) and the FP32 multi-tile kernel for long reads.  Plain:
+// one exact launch per non-empty bin.  Batched: one persistent launch per
+// kernel template covering all of its bins, larger tiles first.
+int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
+{
+    const int ntiles = (int)tiles.size();
+    const int mb = 2 * ntiles;  // multi-tile bin
+    float* spill32 = static_cast<float*>(spill.p);
+    const int64_t n32 = (int64_t)(spill.cap / sizeof(float));
+    int rc;
+    int ctr = 0;
+    if (plain) {
         for (int b = 0; b < 2 * ntiles; ++b) {
             const int count = h_bin_range[2 * b + 1];
             if (count <= 0)
                 continue;
             const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
             TileLaunch tl;
-            tl.begin = h_bin_range[2 * b];
-            tl.count = count;
             tl.G = tc.lanes;
             tl.bin = b;
-            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
-                return rc;
+            tl.begin = h_bin_range[2 * b];
+            tl.count = count;
             if (b % 2 == 0)
                 stats.fast_alignments += count;
-        }
-        if (h_bin_range[2 * mb + 1] > 0) {
-            stats.multitile_alignments += h_bin_range[2 * mb + 1];
-            if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
-                                                bp.bin_range + 2 * mb + 1, d_hmax, spill32, n32,
-                                                d_error,
-                                                std::min(persist_blocks, h_bin_range[2 * mb + 1]),
-                                                st),
-                            "multi-tile kernel")))
+            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
                 return rc;
         }
     } else {
-        if (timed)
-            hipEventRecord(evp.first, st);
-        for (int b = 0; b < 2 * ntiles; ++b) {
-            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            TileLaunch tl;
-            tl.begin_ptr = bp.bin_range + 2 * b;
-            tl.count_ptr = bp.bin_range + 2 * b + 1;
-            tl.G = tc.lanes;
-            tl.bin = b;
-            tl.max_blocks = persist_blocks;
-            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
-                return rc;
-        }
-        if ((rc = check(launch_fwd_wave_f32(fp, nullptr, bp.bin_range + 2 * mb,
-                                            bp.bin_range + 2 * mb + 1, d_hmax, spill32, n32,
-                                            d_error, persist_blocks, st),
-                        "multi-tile kernel")))
-            return rc;
-    }
-    if (timed) {
-        hipEventRecord(evp.second, st);
-        ev_pending.push_back(evp);
-    }
-    // pass 2: FP64 recompute of each bin's rescue list with an FP64 tile of
-    // at least the bin's size (tiles above 512 rows: one wave per alignment);
-    // pass 3: FP64 waves for anything still not finished
-    if (opt.fp64_rescue) {
-        for (int b = 0; b < 2 * ntiles; ++b) {
-            if (opt.mode == PHMM_MODE_PLAIN && h_bin_range[2 * b + 1] <= 0)
+        // templates (rows, form) in the order of their largest bin
+        for (int b0 = 2 * ntiles - 1; b0 >= 0; --b0) {
+            const int rows = tiles[(size_t)(b0 / 2)].rows;
+            const bool fast = (b0 % 2) == 0;
+            bool seen = false;
+            for (int b = b0 + 1; b < 2 * ntiles && !seen; ++b)
+                seen = tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast;
+            if (seen)
                 continue;
-            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            const int tile = tc.max_read_len + 1;  // rows the FP64 tile must hold
-            if (tile <= 512) {
-                const int nr64 = tile <= 64 ? 4 : 8;
-                TileLaunch tl;
-                tl.list = fp.rescue_list;
-                tl.begin_ptr = bp.bin_range + 2 * b;
-                tl.count_ptr = d_rescue_counts + b;
-                tl.G = (tile + nr64 - 1) / nr64;
-                tl.bin = b;
-                tl.max_blocks = num_cus * 2;
-                if ((rc = check(launch_fwd_tile_f64(nr64, fp, tl, st), "FP64 tile kernel")))
-                    return rc;
-            } else if ((rc = check(launch_fwd_wave_f64(fp, fp.rescue_list, bp.bin_range + 2 * b,
-                                                       d_rescue_counts + b, d_hmax, spill64, n64,
-                                                       d_error, num_cus * 2, st),
-                                   "FP64 wave kernel"))) {
+            BinSet bs;
+            for (int b = b0; b >= 0; --b)
+                if (tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast) {
+                    bs.bin[bs.n] = b;
+                    bs.lanes[bs.n] = tiles[(size_t)(b / 2)].lanes;
+                    ++bs.n;
+                }
+            bs.range = bp.bin_range;
+            bs.counts = bp.bin_range + 1;
+            bs.count_stride = 2;
+            if (ctr >= kTaskCounters)
+                return set_error(PHMM_ERANGE, "too many set launches");
+            bs.next_task = bp.task_ctr + ctr++;
+            if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st), "forward kernel")))
                 return rc;
-            }
         }
-        if ((rc = check(launch_fwd_wave_f64(fp, fp.final_list, nullptr, d_final_count, d_hmax,
-                                            spill64, n64, d_error, num_cus * 2, st),
-                        "FP64 final kernel")) ||
-            (rc = check(launch_add_count(d_rescued_total, d_rescue_counts, 2 * ntiles, st),
-                        "rescue count")) ||
-            (rc = check(launch_add_count(d_rescued_total, d_final_count, 1, st), "rescue count")))
-            return rc;
     }
-    stats.alignments += nA;
+    if (plain) {
+        const int n = h_bin_range[2 * mb + 1];
+        if (n <= 0)
+            return PHMM_OK;
+        stats.multitile_alignments += n;
+    }
+    BinSet ws;
+    ws.n = 1;
+    ws.bin[0] = mb;
+    ws.lanes[0] = 64;
+    ws.range = bp.bin_range;
+    ws.counts = bp.bin_range + 1;
+    ws.count_stride = 2;
+    const int blocks = plain ? std::min(num_cus * 8, h_bin_range[2 * mb + 1]) : num_cus * 8;
+    return check(launch_fwd_wave_f32(fp, ws, bp.hmax, spill32, n32, bp.error_flag, blocks, st),
+                 "multi-tile kernel");
+}
+
+// Pass 2: FP64 recompute of each bin's rescue list with an FP64 tile of at
+// least the bin's size (tiles above 512 rows: one wave per alignment), one
+// launch per FP64 template; pass 3: FP64 waves for anything still not
+// finished.
+int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
+{
+    const int ntiles = (int)tiles.size();
+    double* spill64 = static_cast<double*>(spill.p);
+    const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
+    int rc;
+    BinSet set4, set8, wave;
+    for (BinSet* x : {&set4, &set8, &wave}) {
+        x->list = fp.rescue_list;
+        x->range = bp.bin_range;
+        x->counts = fp.rescue_counts;
+        x->count_stride = 1;
+    }
+    for (int b = 2 * ntiles - 1; b >= 0; --b) {
+        if (plain && h_bin_range[2 * b + 1] <= 0)
+            continue;
+        const int tile = tiles[(size_t)(b / 2)].max_read_len + 1;  // rows the FP64 tile must hold
+        BinSet& x = tile > 512 ? wave : tile <= 64 ? set4 : set8;
+        const int nr64 = tile <= 64 ? 4 : 8;
+        x.bin[x.n] = b;
+        x.lanes[x.n] = tile > 512 ? 64 : (tile + nr64 - 1) / nr64;
+        ++x.n;
+    }
+    set8.next_task = bp.task_ctr + (kTaskCounters - 2);
+    set4.next_task = bp.task_ctr + (kTaskCounters - 1);
+    if ((set8.n && (rc = check(launch_fwd_tile_set_f64(8, fp, set8, 0, st), "FP64 tile kernel"))) ||
+        (set4.n && (rc = check(launch_fwd_tile_set_f64(4, fp, set4, 0, st), "FP64 tile kernel"))) ||
+        (wave.n && (rc = check(launch_fwd_wave_f64(fp, wave, bp.hmax, spill64, n64, bp.error_flag,
+                                                   num_cus * 8, st),
+                               "FP64 wave kernel"))))
+        return rc;
+    BinSet fin;
+    fin.n = 1;
+    fin.bin[0] = 0;
+    fin.lanes[0] = 64;
+    fin.list = fp.final_list;
+    fin.counts = fp.final_count;
+    fin.count_stride = 0;
+    if ((rc = check(launch_fwd_wave_f64(fp, fin, bp.hmax, spill64, n64, bp.error_flag, num_cus * 8,
+                                        st),
+                    "FP64 final kernel")) ||
+        (rc = check(launch_add_count(d_rescued_total, fp.rescue_counts, 2 * ntiles, st),
+                    "rescue count")) ||
+        (rc = check(launch_add_count(d_rescued_total, fp.final_count, 1, st), "rescue count")))
+        return rc;
     return PHMM_OK;
+}
+
+// Captures the given stages of the batched sequence (fixed grids) on a
+// private stream and instantiates the graph.
+int Engine::capture(int stages, const BinParams& bp, const FwdParams& fp, const BatchDesc& hd,
+                    hipGraphExec_t* out)
+{
+    int rc;
+    if (!cap_stream &&
+        (rc = check(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "stream")))
+        return rc;
+    if ((rc = check(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeRelaxed),
+                    "begin capture")))
+        return rc;
+    int r2 = PHMM_OK;
+    if (stages & kStagePrep)
+        r2 = enqueue_prep(bp, hd, true, cap_stream);
+    if (!r2 && (stages & kStageFwd))
+        r2 = enqueue_fwd(fp, bp, false, cap_stream);
+    if (!r2 && (stages & kStageRescue) && opt.fp64_rescue)
+        r2 = enqueue_rescue(fp, bp, false, cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap_stream, &g);
+    if (r2) {
+        if (g)
+            hipGraphDestroy(g);
+        return r2;
+    }
+    if ((rc = check(e, "end capture")))
+        return rc;
+    const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e2 != hipSuccess)
+        *out = nullptr;
+    ++graph_captures;
+    return check(e2, "graph instantiate");
+}
+
+void Engine::drop_graphs()
+{
+    hipGraphExec_t* all[] = {&graphs.all, &graphs.prep, &graphs.fwd, &graphs.rescue};
+    bool any = false;
+    for (auto* x : all)
+        any = any || *x;
+    if (any)
+        hipDeviceSynchronize();  // launched replays may still run
+    for (auto* x : all)
+        if (*x) {
+            hipGraphExecDestroy(*x);
+            *x = nullptr;
+        }
 }
 
 void Engine::free_slot(Slot& s)
@@ -627,6 +799,7 @@ extern "C" void phmm_options_default(phmm_options* o)
     o->fp64_rescue = 1;
     o->rescue_min_sum = 1e-28f;
     o->verbose = 0;
+    o->graphs = 1;
 }
 
 extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engine** out)
@@ -714,6 +887,8 @@ extern "C" int phmm_engine_stats(const phmm_engine* h, phmm_stats* out)
     }
     *out = h->e.stats;
     out->rescued = rescued;
+    out->graph_launches = h->e.graph_launches;
+    out->graph_captures = h->e.graph_captures;
     return PHMM_OK;
 }
 
@@ -724,6 +899,8 @@ extern "C" int phmm_engine_reset_stats(phmm_engine* h)
     std::lock_guard<std::mutex> lk(h->mu);
     h->e.finish_timing();
     h->e.stats = phmm_stats{};
+    h->e.graph_launches = 0;
+    h->e.graph_captures = 0;
     if (h->e.d_rescued_total) {
         hipSetDevice(h->e.device);
         hipDeviceSynchronize();
@@ -749,6 +926,7 @@ extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, 
     }
     std::lock_guard<std::mutex> lk(h->mu);
     h->e.tiles.assign(t, t + n);
+    ++h->e.generation;  // captured graphs hold the old table
     return PHMM_OK;
 }
 

@@ -46,6 +46,30 @@ struct Engine {
 
     int64_t* d_rescued_total = nullptr;  // cumulative FP64 rescues
 
+    // The per-call batch descriptor: one device copy, staged through a ring
+    // of pinned slots (a slot is rewritten only after its copy ran).
+    static constexpr int kDescRing = 16;
+    DevBuf desc;
+    BatchDesc* h_desc = nullptr;
+    hipEvent_t desc_ev[kDescRing] = {};
+    bool desc_used[kDescRing] = {};
+    int desc_next = 0;
+    // Calls share the scratch buffers: a call on another stream than the
+    // previous one waits for it.
+    hipEvent_t call_done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool call_recorded = false;
+    // Captured graphs of the batched kernel sequence.  Kernel arguments hold
+    // scratch pointers and the tile table, so every reallocation or table
+    // change starts a new generation and the graphs are captured again.
+    uint64_t generation = 0;
+    struct Graphs {
+        uint64_t gen = ~0ull;
+        hipGraphExec_t all = nullptr, prep = nullptr, fwd = nullptr, rescue = nullptr;
+    } graphs;
+    hipStream_t cap_stream = nullptr;
+    int64_t graph_captures = 0, graph_launches = 0;
+
     // host pipeline: slots of pinned staging + device copies
     struct Slot {
         PinBuf pin;
@@ -75,6 +99,16 @@ struct Engine {
     int ensure_pin(PinBuf& b, size_t bytes, const char* what);
     int ensure_zeroed(DevBuf& b, size_t bytes, const char* what);
     int score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st);
+    // The three stages of a call: binning, FP32 forward kernels, FP64
+    // rescue.  `plain`: sizes read back to the host and exact launches;
+    // else persistent grids that read their ranges from device memory.
+    int enqueue_prep(const BinParams& bp, const BatchDesc& hd, bool fixed, hipStream_t st);
+    int enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st);
+    int enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st);
+    enum { kStagePrep = 1, kStageFwd = 2, kStageRescue = 4 };
+    int capture(int stages, const BinParams& bp, const FwdParams& fp, const BatchDesc& hd,
+                hipGraphExec_t* out);
+    void drop_graphs();
     int score_batch(const phmm_batch_view& v, float* out);
     int finish_timing();
     int take_error_flag();

@@ -122,6 +122,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                                               const int32_t* list, int begin, int count, int G,
                                               int task, int bin)
 {
+    const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
     using V = typename TT::V;
     constexpr int K = TT::K;
@@ -141,16 +142,16 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
         idx = list ? list[begin + local] : begin + local;
         const WorkItem it = p.items[idx];
         out = it.out;
-        L = p.read_len[it.read];
-        HL = p.hap_len[it.hap];
-        roff = p.read_off[it.read];
-        hoff = p.hap_off[it.hap];
+        L = bd.read_len[it.read];
+        HL = bd.hap_len[it.hap];
+        roff = bd.read_off[it.read];
+        hoff = bd.hap_off[it.hap];
         if (FAST)
             nl = p.meta[it.read].nl;
     }
     T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
     if (p.semantics == 1 && active && L > 0) {
-        const float e = ph2pr_of(p.ph2pr, p.gcp[roff]);
+        const float e = ph2pr_of(p.ph2pr, bd.gcp[roff]);
         eps = (T)e;
         beta = F32 ? (T)(float)(1.0 - (double)e) : (T)1 - (T)e;
     }
@@ -348,14 +349,14 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 score = log10f(a) - (kLog10Scale + log10f((float)beta));
                 raw = a / (float)beta;
             }
-            p.scores[out] = score;
+            bd.scores[out] = score;
             if (p.rescue_on && !(raw >= p.rescue_min)) {
                 const int n = atomicAdd(p.rescue_counts + bin, 1);
                 p.rescue_list[begin + n] = idx;
             }
         } else {
             const double a = (double)acc;
-            p.scores[out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
+            bd.scores[out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
             if (!(a > 0.0 && a < 1e300)) {
                 const int n = atomicAdd(p.final_count, 1);
                 p.final_list[n] = idx;
@@ -384,6 +385,77 @@ __global__ __launch_bounds__(64) void k_fwd_tile(FwdParams p, const int32_t* lis
         fwd_tile_task<T, NR, FAST>(p, tab, list, begin, count, G, task, bin);
 }
 
+// Lane i < bs.n of a wave describes bin i of a BinSet: its id, lanes per
+// alignment, begin, count and number of tasks (`per_task` items per task,
+// 0: 64 / lanes alignments per task); incl is the inclusive prefix of tasks
+// over the lanes.  Returns the total number of tasks.
+struct SetLane {
+    int bin, G, begin, count, ntask, incl;
+};
+__device__ __forceinline__ int set_lanes(const BinSet& bs, int per_task, SetLane& s)
+{
+    const int lane = threadIdx.x;
+    s.bin = 0;
+    s.G = 1;
+    s.begin = 0;
+    s.count = 0;
+    s.ntask = 0;
+    if (lane < bs.n) {
+        s.bin = bs.bin[lane];
+        s.G = bs.lanes[lane] > 0 ? bs.lanes[lane] : 1;
+        s.count = bs.counts[(int64_t)bs.count_stride * s.bin];
+        s.begin = bs.range ? bs.range[2 * s.bin] : 0;
+        const int per = per_task > 0 ? per_task : 64 / s.G;
+        s.ntask = (s.count + per - 1) / per;
+    }
+    int x = s.ntask;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    s.incl = x;
+    return __shfl(x, 63, 64);
+}
+
+// The lane (bin) whose task range holds `task` (task < total).
+__device__ __forceinline__ int set_find(const SetLane& s, int task)
+{
+    const unsigned long long m = __ballot(s.incl > task);
+    return __ffsll((long long)m) - 1;
+}
+
+// Next task of a wave from the launch's counter (one atomic per task).
+__device__ __forceinline__ int next_task(int32_t* ctr)
+{
+    int t = 0;
+    if (threadIdx.x == 0)
+        t = atomicAdd(ctr, 1);
+    return __shfl(t, 0, 64);
+}
+
+// Bins of one template in one persistent launch.  Waves take tasks in order
+// from a counter: consecutive tasks of a length-sorted bin have similar
+// lengths, and a wave that finishes early takes the next one.
+template <typename T, int NR, bool FAST>
+__global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
+{
+    static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
+    __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
+    SetLane sl;
+    const int total = set_lanes(bs, 0, sl);
+    for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
+        const int k = set_find(sl, task);
+        const int bin = __shfl(sl.bin, k, 64);
+        const int G = __shfl(sl.G, k, 64);
+        const int begin = __shfl(sl.begin, k, 64);
+        const int count = __shfl(sl.count, k, 64);
+        const int t0 = __shfl(sl.incl - sl.ntask, k, 64);
+        fwd_tile_task<T, NR, FAST>(p, tab, bs.list, begin, count, G, task - t0, bin);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_fwd_wave
 // ---------------------------------------------------------------------------
@@ -402,8 +474,8 @@ struct WaveScalar<double> {
     __device__ static int ilogb(double v) { return ::ilogb(v); }
 };
 
-// Item w of the work is items[list ? list[begin + w] : begin + w] for
-// w < *count_ptr, begin = begin_ptr ? *begin_ptr : 0.  `spill` holds
+// The items are those of the BinSet, one wave each (blocks take every
+// gridDim-th item of the bins' concatenated item sequence).  `spill` holds
 // spill_elems values of T; each resident block holds the border row's M, I,
 // D per column and the power-of-two exponents they were stored with.
 //
@@ -417,18 +489,17 @@ struct WaveScalar<double> {
 // threshold (as the tile kernels define it) to the final list, which the
 // T = double instance recomputes.
 template <typename T, int NR>
-__global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* list,
-                                                 const int32_t* begin_ptr,
-                                                 const int32_t* count_ptr, const int32_t* hmax_ptr,
+__global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const int32_t* hmax_ptr,
                                                  T* spill, int64_t spill_elems,
                                                  int32_t* error_flag)
 {
     using S = WaveScalar<T>;
+    const BatchDesc& bd = *p.bd;
     constexpr int TR = 64 * NR;
     constexpr int kNone = -(1 << 28);
     const int lane = threadIdx.x;
-    const int n = *count_ptr;
-    const int begin = begin_ptr ? *begin_ptr : 0;
+    SetLane sl;
+    const int n = set_lanes(bs, 1, sl);  // one item per wave
     const int hmax = *hmax_ptr;
     // per block: 3*(hmax+1) values of T, then 2*(hmax+1) int32 exponents,
     // in units of T (rounded up to 16 bytes)
@@ -449,27 +520,30 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
     int32_t* ex = reinterpret_cast<int32_t*>(sp + 3 * (int64_t)(hmax + 1));
 
     for (int w = blockIdx.x; w < n; w += nblk) {
-        const int idx = list ? list[begin + w] : begin + w;
+        const int kb = set_find(sl, w);
+        const int begin = __shfl(sl.begin, kb, 64);
+        const int w0 = __shfl(sl.incl - sl.ntask, kb, 64);
+        const int idx = bs.list ? bs.list[begin + (w - w0)] : begin + (w - w0);
         const WorkItem it = p.items[idx];
-        const int L = p.read_len[it.read];
-        const int HL = p.hap_len[it.hap];
-        const int roff = p.read_off[it.read];
-        const int hoff = p.hap_off[it.hap];
+        const int L = bd.read_len[it.read];
+        const int HL = bd.hap_len[it.hap];
+        const int roff = bd.read_off[it.read];
+        const int hoff = bd.hap_off[it.hap];
         if (L <= 0 || HL <= 0) {
             if (lane == 0)
-                p.scores[it.out] = -INFINITY;
+                bd.scores[it.out] = -INFINITY;
             continue;
         }
         if (HL > hmax) {
             if (lane == 0) {
                 atomicOr(error_flag, 2);
-                p.scores[it.out] = __int_as_float(0x7fc00000);
+                bd.scores[it.out] = __int_as_float(0x7fc00000);
             }
             continue;
         }
         T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
         if (p.semantics == 1) {
-            const float e = ph2pr_of(p.ph2pr, p.gcp[roff]);
+            const float e = ph2pr_of(p.ph2pr, bd.gcp[roff]);
             eps = (T)e;
             beta = (T)1 - (T)e;
         }
@@ -621,7 +695,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, const int32_t* lis
         if (lane == tr) {
             const double lg = log10((double)acc) + (double)sc * 0.30102999566398119521 -
                               log10((double)C0 * (double)HL);
-            p.scores[it.out] = (float)lg;
+            bd.scores[it.out] = (float)lg;
             // same criterion as the tile kernels: the sum in the reference's
             // units (start FLT_MAX/16/H) below rescue_min, or not finite
             const double lmin = log10((double)p.rescue_min) - (double)kLog10Scale;
@@ -678,34 +752,125 @@ hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaun
     }
 }
 
-hipError_t launch_fwd_tile_f64(int NR, const FwdParams& p, const TileLaunch& tl, hipStream_t st)
+template <typename K>
+static int resident_blocks_per_cu(K kernel)
 {
-    if (tl.G < 1 || tl.G > 64)
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 64, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    return n > 0 ? n : 4;
+}
+
+template <typename T, int NR, bool FAST>
+static int set_blocks_per_cu()
+{
+    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST>);
+    return n;
+}
+
+static int grid_of(int grid, int per_cu)
+{
+    if (grid > 0)
+        return grid;
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        else {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+    }
+    return cus * per_cu;
+}
+
+template <typename T, int NR>
+static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs, int grid,
+                                hipStream_t st)
+{
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.next_task)
         return hipErrorInvalidValue;
+    for (int i = 0; i < bs.n; ++i)
+        if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
+            return hipErrorInvalidValue;
+    if (fast && sizeof(T) == 4) {
+        hipLaunchKernelGGL((k_fwd_tile_set<float, NR, true>),
+                           dim3(grid_of(grid, set_blocks_per_cu<float, NR, true>())), dim3(64), 0,
+                           st, p, bs);
+    } else {
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false>())), dim3(64), 0, st,
+                           p, bs);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,
+                               hipStream_t st)
+{
     switch (NR) {
-    case 4: return launch_tile_nr<double, 4>(false, p, tl, st);
-    case 8: return launch_tile_nr<double, 8>(false, p, tl, st);
+    case 4: return launch_set_nr<float, 4>(fast, p, bs, grid, st);
+    case 8: return launch_set_nr<float, 8>(fast, p, bs, grid, st);
+    case 12: return launch_set_nr<float, 12>(fast, p, bs, grid, st);
+    case 16: return launch_set_nr<float, 16>(fast, p, bs, grid, st);
+    case 20: return launch_set_nr<float, 20>(fast, p, bs, grid, st);
+    case 24: return launch_set_nr<float, 24>(fast, p, bs, grid, st);
+    case 28: return launch_set_nr<float, 28>(fast, p, bs, grid, st);
+    case 32: return launch_set_nr<float, 32>(fast, p, bs, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_fwd_wave_f32(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
-                               const int32_t* count_ptr, const int32_t* hmax_ptr, float* spill,
-                               int64_t spill_elems, int32_t* error_flag, int blocks,
+hipError_t launch_fwd_tile_set_f64(int NR, const FwdParams& p, const BinSet& bs, int grid,
+                                   hipStream_t st)
+{
+    switch (NR) {
+    case 4: return launch_set_nr<double, 4>(false, p, bs, grid, st);
+    case 8: return launch_set_nr<double, 8>(false, p, bs, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64)
+{
+    if (f64)
+        return NR == 4 ? set_blocks_per_cu<double, 4, false>() : set_blocks_per_cu<double, 8, false>();
+    switch (NR) {
+    case 4: return fast ? set_blocks_per_cu<float, 4, true>() : set_blocks_per_cu<float, 4, false>();
+    case 8: return fast ? set_blocks_per_cu<float, 8, true>() : set_blocks_per_cu<float, 8, false>();
+    case 12: return fast ? set_blocks_per_cu<float, 12, true>() : set_blocks_per_cu<float, 12, false>();
+    case 16: return fast ? set_blocks_per_cu<float, 16, true>() : set_blocks_per_cu<float, 16, false>();
+    case 20: return fast ? set_blocks_per_cu<float, 20, true>() : set_blocks_per_cu<float, 20, false>();
+    case 24: return fast ? set_blocks_per_cu<float, 24, true>() : set_blocks_per_cu<float, 24, false>();
+    case 28: return fast ? set_blocks_per_cu<float, 28, true>() : set_blocks_per_cu<float, 28, false>();
+    case 32: return fast ? set_blocks_per_cu<float, 32, true>() : set_blocks_per_cu<float, 32, false>();
+    default: return 0;
+    }
+}
+
+hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
+                               float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fwd_wave<float, 16>), dim3(blocks), dim3(64), 0, st, p, list, begin_ptr,
-                       count_ptr, hmax_ptr, spill, spill_elems, error_flag);
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || blocks <= 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fwd_wave<float, 16>), dim3(blocks), dim3(64), 0, st, p, bs, hmax_ptr,
+                       spill, spill_elems, error_flag);
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_wave_f64(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
-                               const int32_t* count_ptr, const int32_t* hmax_ptr, double* spill,
-                               int64_t spill_elems, int32_t* error_flag, int blocks,
+hipError_t launch_fwd_wave_f64(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
+                               double* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fwd_wave<double, 8>), dim3(blocks), dim3(64), 0, st, p, list, begin_ptr,
-                       count_ptr, hmax_ptr, spill, spill_elems, error_flag);
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || blocks <= 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fwd_wave<double, 8>), dim3(blocks), dim3(64), 0, st, p, bs, hmax_ptr,
+                       spill, spill_elems, error_flag);
     return hipGetLastError();
 }
 

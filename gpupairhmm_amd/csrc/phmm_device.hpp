@@ -33,6 +33,7 @@ constexpr double kStartF64 = 0x1p1000;
 constexpr double kLog10StartF64 = 301.02999566398120;  // 1000 * log10(2)
 
 constexpr int kMaxTiles = 32;
+constexpr int kTaskCounters = 32;   // set launches per call
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
@@ -68,32 +69,11 @@ struct __attribute__((aligned(8))) ReadMeta {
     float nl;        // product of alpha over the read (rescaled kernel)
 };
 
-// Everything the forward kernels read.
-struct FwdParams {
-    const WorkItem* items;
-    const uint32_t* rows;        // packed read rows, indexed like read bytes
-    const uint8_t* hapc;         // hap base codes, indexed like hap bytes
-    const uint8_t* gcp;          // raw gcp qualities (CPU semantics)
-    const int32_t* read_len;
-    const int32_t* hap_len;
-    const int32_t* read_off;
-    const int32_t* hap_off;
-    const ReadMeta* meta;
-    const float* ph2pr;          // 128 entries
-    float* scores;
-    int32_t* rescue_list;        // per bin: work-item indices to recompute in FP64
-    int32_t* rescue_counts;      // per bin
-    int32_t* final_list;         // items the FP64 tile pass could not finish
-    int32_t* final_count;
-    float rescue_min;            // FP32 sum (reference units) threshold
-    int32_t rescue_on;
-    int32_t semantics;           // 0: reference GPU, 1: reference CPU
-    int32_t pad;
-};
-
-// Everything the binning kernels read and write (binning.hip).
-struct BinParams {
-    // batch (device pointers)
+// The per-call part of a batch: device pointers of a self-contained batch
+// view, the output array and the sizes.  It lives in device memory (one
+// small copy per call), so every kernel launch of a call -- and a captured
+// graph of them -- is independent of the batch.
+struct BatchDesc {
     const uint8_t* read_data;
     const uint8_t* hap_data;
     const uint8_t* bq;
@@ -108,8 +88,33 @@ struct BinParams {
     const int32_t* nhaps;      // per group
     const int32_t* rprefix;    // [G+1]
     const int32_t* hprefix;    // [G+1]
-    int32_t G, nR, nH, NB;
-    int64_t read_bytes, hap_bytes;
+    float* scores;
+    int32_t G, nR, nH, pad;
+    int64_t read_bytes, hap_bytes, num_alignments;
+};
+
+// Everything the forward kernels read besides the batch descriptor.
+struct FwdParams {
+    const BatchDesc* bd;
+    const WorkItem* items;
+    const uint32_t* rows;        // packed read rows, indexed like read bytes
+    const uint8_t* hapc;         // hap base codes, indexed like hap bytes
+    const ReadMeta* meta;
+    const float* ph2pr;          // 128 entries
+    int32_t* rescue_list;        // per bin: work-item indices to recompute in FP64
+    int32_t* rescue_counts;      // per bin
+    int32_t* final_list;         // items the FP64 tile pass could not finish
+    int32_t* final_count;
+    float rescue_min;            // FP32 sum (reference units) threshold
+    int32_t rescue_on;
+    int32_t semantics;           // 0: reference GPU, 1: reference CPU
+    int32_t pad;
+};
+
+// Everything the binning kernels read and write (binning.hip).
+struct BinParams {
+    const BatchDesc* bd;
+    int32_t NB;
     // scratch
     uint32_t* rows;
     uint8_t* hapc;
@@ -125,8 +130,8 @@ struct BinParams {
     int32_t* rescue_counts;    // [NB]
     int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
+    int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
     WorkItem* items;
-    float* scores;
     const float* ph2pr;
     // tiles
     int32_t ntiles;
@@ -135,9 +140,12 @@ struct BinParams {
     int32_t allow_fast;
 };
 
-hipError_t run_binning(const BinParams& bp, hipStream_t st);
+// Binning launches.  `hd` is the host copy of the descriptor bp.bd points to;
+// with fixed_grids every grid is independent of the batch sizes (graph
+// capture), else grids are sized to them.
+hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, hipStream_t st);
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
-// One launch of k_fwd_tile over one bin (or one bin's rescue list).
+// One launch of k_fwd_tile over one bin (exact grid; plain mode).
 struct TileLaunch {
     const int32_t* list = nullptr;       // item indices, or null: items are contiguous
     const int32_t* begin_ptr = nullptr;  // device begin/count (null: host values)
@@ -149,14 +157,35 @@ struct TileLaunch {
 };
 hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaunch& tl,
                            hipStream_t st);
-hipError_t launch_fwd_tile_f64(int NR, const FwdParams& p, const TileLaunch& tl, hipStream_t st);
-hipError_t launch_fwd_wave_f32(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
-                               const int32_t* count_ptr, const int32_t* hmax_ptr, float* spill,
-                               int64_t spill_elems, int32_t* error_flag, int blocks,
+
+// Several bins in one persistent launch: the work of bin bin[i] is
+// list ? list[begin_i + w] : begin_i + w for w < count_i, with
+// begin_i = range ? range[2*bin[i]] : 0 and count_i =
+// counts[count_stride*bin[i]], both read on the device.  The bins' tasks
+// form one sequence in the given order; waves take the next task from
+// *next_task (zeroed before the launch).
+struct BinSet {
+    int32_t n = 0;
+    int32_t bin[2 * kMaxTiles + 1] = {};
+    int32_t lanes[2 * kMaxTiles + 1] = {};  // tile kernels: lanes per alignment
+    const int32_t* list = nullptr;
+    const int32_t* range = nullptr;
+    const int32_t* counts = nullptr;
+    int32_t count_stride = 0;
+    int32_t* next_task = nullptr;
+};
+// grid: 0 = as many blocks as fit on the device at once
+hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                hipStream_t st);
-hipError_t launch_fwd_wave_f64(const FwdParams& p, const int32_t* list, const int32_t* begin_ptr,
-                               const int32_t* count_ptr, const int32_t* hmax_ptr, double* spill,
-                               int64_t spill_elems, int32_t* error_flag, int blocks,
+hipError_t launch_fwd_tile_set_f64(int NR, const FwdParams& p, const BinSet& bs, int grid,
+                                   hipStream_t st);
+hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
+                               float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
+hipError_t launch_fwd_wave_f64(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
+                               double* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
+                               hipStream_t st);
+// Resident blocks of 64 threads per CU for a tile-set instance.
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64);
 
 }  // namespace phmm

@@ -74,7 +74,10 @@ typedef struct phmm_options {
                                   in FP64; default 1e-28 */
     int32_t verbose;
     int32_t time_kernels;      /* 1: event-time the forward kernels (stats) */
-    int32_t reserved[6];
+    int32_t graphs;            /* PHMM_MODE_BATCHED: 1 (default) replays the
+                                  call's kernel sequence as one captured HIP
+                                  graph; 0 launches the kernels one by one */
+    int32_t reserved[5];
 } phmm_options;
 
 /* ---- batch view (BatchOfSequenceGroups, common.cuh:109-201) --------- *
@@ -124,7 +127,10 @@ typedef struct phmm_stats {
     int64_t multitile_alignments;
     double engine_seconds;     /* wall time inside phmm_score_batch */
     double kernel_seconds;     /* event-timed forward kernels (if timed) */
-    int64_t reserved[8];
+    int64_t graph_launches;    /* HIP graph replays (batched mode) */
+    int64_t graph_captures;    /* graphs captured (first call, new scratch
+                                  sizes or a new tile table) */
+    int64_t reserved[6];
 } phmm_stats;
 
 /* One row of the tile table (the reference's per-architecture
