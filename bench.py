@@ -138,7 +138,7 @@ def bench_full_mixed(G, records, seed):
 
 def bench_precision(G, records, seed):
     """Configuration 3: FP32 bulk with FP64 recompute of the alignments whose
-    FP32 sum falls below 1e-28 (reference units); rescued fraction and
+    FP32 sum falls below FLT_MIN (reference units); rescued fraction and
     parity on a 2% sample."""
     batch = G.HostBatch.synthetic(3, seed=seed, num_records=records)
     cells = batch.cells()

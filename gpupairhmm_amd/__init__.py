@@ -68,6 +68,11 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
 
+# Smallest normal float: the default rescue threshold (the FP32 sum in the
+# reference's units below which a result is recomputed in FP64).
+FLT_MIN = 1.1754943508222875e-38
+
+
 class Options(ctypes.Structure):
     _fields_ = [
         ("transfer_batchsize", ctypes.c_int64), ("mode", ctypes.c_int32),
@@ -484,7 +489,7 @@ class Engine:
 
     def __init__(self, device: int = 0, mode: int = MODE_BATCHED, semantics: int = SEM_GPU,
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
-                 rescue_min_sum: float = 1e-28, time_kernels: bool = False,
+                 rescue_min_sum: float = FLT_MIN, time_kernels: bool = False,
                  graphs: bool = True):
         lib = load_library()
         o = Options()

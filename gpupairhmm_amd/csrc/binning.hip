@@ -65,60 +65,65 @@ __global__ void k_hap_max(BinParams bp)
         atomicMax(bp.hmax, m);
 }
 
-// Per read: its bin.  Tile class k is the first table row whose
-// max_read_len covers the read (its tile has room for one more row); reads
-// beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses the
-// rescaled (FAST) arithmetic when the read's coefficients keep it in range,
-// else bin 2k+1 (SAFE).
-__global__ void k_read_meta(BinParams bp)
+// Per read (one wave each): its bin.  Tile class k is the first table row
+// whose max_read_len covers the read (its tile has room for one more row);
+// reads beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses
+// the rescaled (FAST) arithmetic when the read's coefficients keep it in
+// range, else bin 2k+1 (SAFE).  The lanes check rows lane, lane+64, ... and
+// the wave combines their verdicts and the product of alpha.
+__global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    GRID_STRIDE(i, b.nR)
-    {
+    const int lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < b.nR; i += nwaves) {
         const int L = b.read_len[i];
         ReadMeta m;
         m.nl = 1.0f;
-        if (L <= 0) {
-            m.bin = -1;
-            bp.meta[i] = m;
-            continue;
-        }
         int k = bp.ntiles;
         for (int j = 0; j < bp.ntiles; ++j)
             if (L <= bp.tiles[j]) {
                 k = j;
                 break;
             }
-        if (k == bp.ntiles) {
-            m.bin = 2 * bp.ntiles;
-            bp.meta[i] = m;
+        if (L <= 0 || k == bp.ntiles) {
+            m.bin = L <= 0 ? -1 : 2 * bp.ntiles;
+            if (lane == 0)
+                bp.meta[i] = m;
             continue;
         }
         const int off = b.read_off[i];
         float eps = kEpsGpu;
         if (bp.semantics == 1)
             eps = bp.ph2pr[min((int)b.gcp[off], 127)];
-        bool ok = bp.allow_fast != 0 && eps <= 0.5f;
+        bool ok = true;
         double nl = 1.0;
-        float pdelta = 0.f, palpha = 1.f;
-        for (int r = 0; r < L && ok; ++r) {
+        for (int r = lane; r < L; r += 64) {
             const uint32_t w = bp.rows[off + r];
             const float delta = bp.ph2pr[row_iq(w)];
             const float sigma = bp.ph2pr[row_dq(w)];
             const float alpha = 1.0f - (delta + sigma);
-            if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
+            if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f)) {
                 ok = false;
-            else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
-                ok = false;
+            } else if (r > 0) {
+                const uint32_t wp = bp.rows[off + r - 1];
+                const float pdelta = bp.ph2pr[row_iq(wp)];
+                const float palpha = 1.0f - (pdelta + bp.ph2pr[row_dq(wp)]);
+                if (!(eps * pdelta / (delta * palpha) <= 0.5f))
+                    ok = false;
+            }
             nl *= (double)alpha;
-            pdelta = delta;
-            palpha = alpha;
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            nl *= __shfl_xor(nl, o, 64);
+        ok = __all(ok) && bp.allow_fast != 0 && eps <= 0.5f;
         if (ok && !(nl >= 1e-30))  // the start constant must stay a normal float
             ok = false;
         m.nl = ok ? (float)nl : 1.0f;
         m.bin = 2 * k + (ok ? 0 : 1);
-        bp.meta[i] = m;
+        if (lane == 0)
+            bp.meta[i] = m;
     }
 }
 
@@ -229,52 +234,70 @@ __global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
     }
 }
 
-// Exclusive scan of the key histogram (one block); per-bin ranges.
+// Exclusive scan of each bin's hap-length histogram (one block per bin):
+// key_cursor holds positions relative to the bin's begin, the bin's count
+// goes to bin_range[2*bin+1].
 __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
-    if (threadIdx.x == 0)
-        carry = 0;
-    __syncthreads();
-    const int n = bp.NB * kHapBuckets;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = i < n ? bp.key_count[i] : 0;
+    for (int bin = blockIdx.x; bin < bp.NB; bin += gridDim.x) {
+        if (threadIdx.x == 0)
+            carry = 0;
+        __syncthreads();
+        const int base0 = bin * kHapBuckets;
+        for (int base = 0; base < kHapBuckets; base += 1024) {
+            const int i = base + threadIdx.x;
+            const int v = i < kHapBuckets ? bp.key_count[base0 + i] : 0;
+            int x = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (lane >= o)
+                    x += y;
+            }
+            if (lane == 63)
+                wsum[wv] = x;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int s = 0;
+                for (int k = 0; k < 16; ++k) {
+                    const int t = wsum[k];
+                    wsum[k] = s;
+                    s += t;
+                }
+            }
+            __syncthreads();
+            const int incl = x + wsum[wv] + carry;
+            if (i < kHapBuckets)
+                bp.key_cursor[base0 + i] = incl - v;
+            __syncthreads();
+            if (threadIdx.x == 1023)
+                carry = incl;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            bp.bin_range[2 * bin + 1] = carry;
+        __syncthreads();
+    }
+}
+
+// Bin begins: exclusive scan of the bin counts (NB <= 65, one wave).
+__global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
+{
+    int carry = 0;
+    for (int base = 0; base < bp.NB; base += 64) {
+        const int k = base + (int)threadIdx.x;
+        const int v = k < bp.NB ? bp.bin_range[2 * k + 1] : 0;
         int x = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o, 64);
-            if (lane >= o)
+            if ((int)threadIdx.x >= o)
                 x += y;
         }
-        if (lane == 63)
-            wsum[wv] = x;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int s = 0;
-            for (int k = 0; k < 16; ++k) {
-                const int t = wsum[k];
-                wsum[k] = s;
-                s += t;
-            }
-        }
-        __syncthreads();
-        const int incl = x + wsum[wv] + carry;
-        if (i < n) {
-            bp.key_cursor[i] = incl - v;
-            if (i % kHapBuckets == 0)
-                bp.bin_range[2 * (i / kHapBuckets)] = incl - v;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023)
-            carry = incl;
-        __syncthreads();
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < bp.NB; k += blockDim.x) {
-        const int end = (k + 1 < bp.NB) ? bp.bin_range[2 * (k + 1)] : carry;
-        bp.bin_range[2 * k + 1] = end - bp.bin_range[2 * k];
+        if (k < bp.NB)
+            bp.bin_range[2 * k] = carry + x - v;
+        carry += __shfl(x, 63, 64);
     }
 }
 
@@ -308,7 +331,7 @@ __global__ __launch_bounds__(256) void k_scatter(BinParams bp)
                     }
                     continue;
                 }
-                const int base = atomicAdd(&bp.key_cursor[key_of(k, hl)], n);
+                const int base = bp.bin_range[2 * k] + atomicAdd(&bp.key_cursor[key_of(k, hl)], n);
                 for (int j = 0; j < n; ++j) {
                     const int rd = bp.read_perm[r0 + st + j];
                     WorkItem it;
@@ -364,14 +387,15 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_encode_haps, dim3(grid(hd.hap_bytes, tpb, 2048)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.nH > 0)
         hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.nR > 0)
-        hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb, 2048)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.nR > 0)  // one wave per read
+        hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
         hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
         hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
     }
-    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, bp);
+    hipLaunchKernelGGL(k_scan_keys, dim3(bp.NB), dim3(1024), 0, st, bp);
+    hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(64), 0, st, bp);
     if (fixed_grids || hd.G > 0)
         hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
     return hipGetLastError();

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -797,7 +798,7 @@ extern "C" void phmm_options_default(phmm_options* o)
     o->mode = PHMM_MODE_BATCHED;
     o->semantics = PHMM_SEM_GPU;
     o->fp64_rescue = 1;
-    o->rescue_min_sum = 1e-28f;
+    o->rescue_min_sum = FLT_MIN;  // where the reference's FP32 sum underflows
     o->verbose = 0;
     o->graphs = 1;
 }

@@ -72,6 +72,14 @@ __device__ __forceinline__ int wave_max(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 __device__ __forceinline__ float ph2pr_of(const float* table, int q)
 {
     return table[q > 127 ? 127 : q];
@@ -250,7 +258,12 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
         else
             C0 = (T)(kStartF64 / (double)HL);
     }
-    T Ml = 0, Dl = first ? C0 : (T)0;
+    // boundary injection: a group's first lane takes row 0 (M = I = 0,
+    // D = C0) instead of its left neighbour's last row -- as a multiply by a
+    // 0/1 lane mask, which issues at the full FP32 rate where a select does not
+    const T nf = first ? (T)0 : (T)1;
+    const T c0f = first ? C0 : (T)0;
+    T Ml = 0, Dl = c0f;
     T Mul = 0, Dul = Dl;
     T Iin = 0;
     T acc = 0;
@@ -262,10 +275,12 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
     const int hdw_n = active ? (HL >> 2) + 1 : 0;  // padded length / 4
     uint32_t wcur = (first && 0 < hdw_n) ? hdw[0] : 0x04040404u;
     uint32_t wnext = (first && 1 < hdw_n) ? hdw[1] : 0x04040404u;
-    // the chain is exactly 0 before a lane's first column, so only the end
-    // of the lane's real columns needs a bound: step s adds column s - t,
-    // which is real while s <= t + HL
+    // Step s adds the chain of column s - t.  The chain is exactly 0 before
+    // a lane's first column, so only the end of the lane's real columns,
+    // s <= t + HL, needs a bound: steps below the wave's smallest HL add
+    // unconditionally, the rest compare.
     const int acc_last = t + HL;
+    const int n_free = wave_min(active ? HL : nsteps) + 1;  // steps s < n_free
     const int lane_off = lane;  // tab index of this lane's letter-0 entry
 
     // letter and emission values of step 0; each step loads the next step's
@@ -279,7 +294,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
             lamv[q] = trow[q * 64];
     }
 
-    for (int s4 = 0; s4 < nsteps; s4 += 4) {
+    auto sweep4 = [&](int s4, bool bounded) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             // next step's letter: the group's first lane reads the hap, the
@@ -319,15 +334,18 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                     dp = dov;
                 }
             }
-            acc += (s4 + u <= acc_last) ? chain : (T)0;
+            if (bounded)
+                acc += (s4 + u <= acc_last) ? chain : (T)0;
+            else
+                acc += chain;
             const T sM = TT::shr(M[NR - 1]);
             const T sD = TT::shr(D[NR - 1]);
             const T sI = TT::shr(chain);
             Mul = Ml;
             Dul = Dl;
-            Ml = first ? (T)0 : sM;
-            Dl = first ? C0 : sD;
-            Iin = first ? (T)0 : sI;
+            Ml = sM * nf;
+            Dl = fma(sD, nf, c0f);
+            Iin = sI * nf;
             letter = nletter;
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -336,7 +354,12 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
         const int k = (s4 >> 2) + 2;
         wcur = wnext;
         wnext = (first && k < hdw_n) ? hdw[k] : 0x04040404u;
-    }
+    };
+    int s4 = 0;
+    for (; s4 + 4 <= n_free && s4 < nsteps; s4 += 4)
+        sweep4(s4, false);
+    for (; s4 < nsteps; s4 += 4)
+        sweep4(s4, true);
 
     if (active && t == L / NR) {
         if (F32) {

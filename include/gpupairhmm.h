@@ -71,7 +71,8 @@ typedef struct phmm_options {
     int32_t fp64_rescue;       /* 1: recompute low/invalid FP32 results in FP64 */
     float rescue_min_sum;      /* FP32 likelihood sum (the reference's scaled
                                   units) below which a result is recomputed
-                                  in FP64; default 1e-28 */
+                                  in FP64; default FLT_MIN, the floor of
+                                  the reference's FP32 normal range */
     int32_t verbose;
     int32_t time_kernels;      /* 1: event-time the forward kernels (stats) */
     int32_t graphs;            /* PHMM_MODE_BATCHED: 1 (default) replays the

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import gpupairhmm_amd as G
@@ -39,7 +40,7 @@ def test_abi_version_and_defaults(lib):
     lib.phmm_options_default(ctypes.byref(o))
     assert o.transfer_batchsize == 1000000  # common.cuh:19
     assert o.mode == G.MODE_BATCHED and o.semantics == G.SEM_GPU
-    assert o.fp64_rescue == 1 and abs(o.rescue_min_sum - 1e-28) < 1e-34
+    assert o.fp64_rescue == 1 and o.rescue_min_sum == np.float32(G.FLT_MIN)
 
 
 def test_struct_sizes_match_header(lib):

@@ -152,6 +152,30 @@ def test_fp64_rescue(lib, oracle):
     assert _max_delta(raw, ref) > TOL  # FP32 alone cannot represent these
 
 
+@pytest.mark.parametrize("config,records", [(1, 200), (3, 30)])
+def test_fp32_accurate_above_rescue_threshold(lib, oracle, config, records):
+    """The default rescue threshold is FLT_MIN (the reference's FP32 floor,
+    log10 sum ~ -75.3 in its scaled units).  Without any rescue the FP32
+    kernels must already be accurate for every alignment whose sum stays
+    above it -- with two decades of margin -- and fail below it, which is
+    what the rescue is for."""
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(config, seed=3000 + config, num_records=records)
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    say(f"case: FP32 accuracy above the threshold, config {config}")
+    with G.Engine(fp64_rescue=False) as e:
+        raw = e.score_batch(b)
+    floor = np.log10(G.FLT_MIN) - np.log10(2.1267646664908054e37)  # ~ -75.26
+    above = ref >= floor + 2.0
+    assert above.sum() > 0.5 * len(ref)
+    assert _max_delta(raw[above], ref[above]) <= 1e-4
+    with G.Engine() as e:
+        got = e.score_batch(b)
+        st = e.stats()
+    assert _max_delta(got, ref) <= TOL
+    assert 0 < st.rescued < len(ref)
+
+
 @pytest.mark.parametrize("read_len,hap_len", [(800, 2000), (1300, 1500)])
 def test_fp64_rescue_very_low_scores(lib, oracle, read_len, hap_len):
     """Scores far below the FP32 range (-500 and lower): the FP64 tile pass
