@@ -249,7 +249,7 @@ def main():
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4) if achieved else None,
                 "traffic": traffic,
-                "kernel": "k_fwd_tile (all bins of one step, event-timed)",
+                "kernel": "k_fwd_tile_set (FP32 forward stage of one step, event-timed)",
                 "kernel_ms_per_launch": round(kernel_avg * 1e3, 3) if kernel_avg else None,
                 "flop_per_cell": FLOP_PER_CELL}
     result = {

@@ -115,7 +115,8 @@ class TileConfig(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     _fields_ = [
         ("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
-        ("max_batch_requests", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8),
+        ("max_batch_requests", ctypes.c_int32),
+        ("max_batch_alignments", ctypes.c_int64), ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -596,13 +597,16 @@ class Server:
     """Continuous-batching request loop on an engine."""
 
     def __init__(self, engine: Engine, hbm_budget_bytes: int = 64 << 30,
-                 max_inflight: int = 1 << 20, max_batch_requests: int = 4096):
+                 max_inflight: int = 1 << 20, max_batch_requests: int = 4096,
+                 max_batch_alignments: int | None = None):
         lib = load_library()
         o = ServerOptions()
         lib.phmm_server_options_default(ctypes.byref(o))
         o.hbm_budget_bytes = int(hbm_budget_bytes)
         o.max_inflight = int(max_inflight)
         o.max_batch_requests = int(max_batch_requests)
+        if max_batch_alignments is not None:
+            o.max_batch_alignments = int(max_batch_alignments)
         h = ctypes.c_void_p()
         _check(lib.phmm_server_create(engine._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h

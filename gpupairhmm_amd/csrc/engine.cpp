@@ -576,21 +576,47 @@ void Engine::free_slot(Slot& s)
 
 int Engine::launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1)
 {
+    const ChunkPart part{&v, g0, g1};
+    return launch_parts(s, &part, 1);
+}
+
+// Stages groups of one or more host views, in order, into slot s as one
+// self-contained batch (offsets and prefixes rebased) and enqueues copy-in,
+// scoring and copy-out; s.compute_done fires when s.pin_scores holds the
+// scores.
+int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
+{
+    struct Span {
+        int r0, r1, h0, h1;
+        int64_t rb0, rb1, hb0, hb1;
+    };
+    std::vector<Span> sp((size_t)nparts);
+    int64_t G = 0, nR = 0, nH = 0, rbytes = 0, hbytes = 0, nA = 0;
+    for (int k = 0; k < nparts; ++k) {
+        const phmm_batch_view& v = *parts[k].v;
+        const int g0 = parts[k].g0, g1 = parts[k].g1;
+        Span& x = sp[(size_t)k];
+        x.r0 = v.reads_prefix[g0];
+        x.r1 = v.reads_prefix[g1];
+        x.h0 = v.haps_prefix[g0];
+        x.h1 = v.haps_prefix[g1];
+        x.rb0 = x.r1 > x.r0 ? v.read_offsets[x.r0] : 0;
+        x.rb1 = x.r1 > x.r0 ? v.read_offsets[x.r1 - 1] + padded_length(v.read_lengths[x.r1 - 1]) : 0;
+        x.hb0 = x.h1 > x.h0 ? v.hap_offsets[x.h0] : 0;
+        x.hb1 = x.h1 > x.h0 ? v.hap_offsets[x.h1 - 1] + padded_length(v.hap_lengths[x.h1 - 1]) : 0;
+        G += g1 - g0;
+        nR += x.r1 - x.r0;
+        nH += x.h1 - x.h0;
+        rbytes += x.rb1 - x.rb0;
+        hbytes += x.hb1 - x.hb0;
+        for (int g = g0; g < g1; ++g)
+            nA += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+    }
+    if (G > INT32_MAX || nR > INT32_MAX || nH > INT32_MAX || rbytes > INT32_MAX || hbytes > INT32_MAX)
+        return set_error(PHMM_ERANGE, "transfer chunk exceeds 32-bit offsets; lower transfer_batchsize");
     struct Layout {
         size_t read_data, hap_data, bq, iq, dq, gq, rl, hl, ro, ho, nr, nh, rp, hp, total;
     };
-    const int G = g1 - g0;
-    const int r0 = v.reads_prefix[g0], r1 = v.reads_prefix[g1];
-    const int h0 = v.haps_prefix[g0], h1 = v.haps_prefix[g1];
-    const int nR = r1 - r0, nH = h1 - h0;
-    const int64_t rb0 = nR > 0 ? v.read_offsets[r0] : 0;
-    const int64_t rb1 = nR > 0 ? v.read_offsets[r1 - 1] + padded_length(v.read_lengths[r1 - 1]) : 0;
-    const int64_t hb0 = nH > 0 ? v.hap_offsets[h0] : 0;
-    const int64_t hb1 = nH > 0 ? v.hap_offsets[h1 - 1] + padded_length(v.hap_lengths[h1 - 1]) : 0;
-    const int64_t rbytes = rb1 - rb0, hbytes = hb1 - hb0;
-    int64_t nA = 0;
-    for (int g = g0; g < g1; ++g)
-        nA += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
     Layout L{};
     {
         size_t o = 0;
@@ -621,30 +647,46 @@ int Engine::launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1)
         (rc = ensure_pin(s.pin_scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1),
                          "score staging")))
         return rc;
-    // a self-contained, rebased copy of the chunk
     char* P = static_cast<char*>(s.pin.p);
-    std::memcpy(P + L.read_data, v.read_data + rb0, (size_t)rbytes);
-    std::memcpy(P + L.bq, v.base_quals + rb0, (size_t)rbytes);
-    std::memcpy(P + L.iq, v.ins_quals + rb0, (size_t)rbytes);
-    std::memcpy(P + L.dq, v.del_quals + rb0, (size_t)rbytes);
-    std::memcpy(P + L.gq, v.gcp_quals + rb0, (size_t)rbytes);
-    std::memcpy(P + L.hap_data, v.hap_data + hb0, (size_t)hbytes);
-    std::memcpy(P + L.rl, v.read_lengths + r0, 4 * (size_t)nR);
-    std::memcpy(P + L.hl, v.hap_lengths + h0, 4 * (size_t)nH);
     int32_t* ro = reinterpret_cast<int32_t*>(P + L.ro);
-    for (int i = 0; i <= nR; ++i)
-        ro[i] = (int32_t)((i < nR ? v.read_offsets[r0 + i] : rb1) - rb0);
     int32_t* ho = reinterpret_cast<int32_t*>(P + L.ho);
-    for (int i = 0; i <= nH; ++i)
-        ho[i] = (int32_t)((i < nH ? v.hap_offsets[h0 + i] : hb1) - hb0);
-    std::memcpy(P + L.nr, v.num_reads_per_group + g0, 4 * (size_t)G);
-    std::memcpy(P + L.nh, v.num_haps_per_group + g0, 4 * (size_t)G);
     int32_t* rp = reinterpret_cast<int32_t*>(P + L.rp);
     int32_t* hp = reinterpret_cast<int32_t*>(P + L.hp);
-    for (int g = 0; g <= G; ++g) {
-        rp[g] = v.reads_prefix[g0 + g] - r0;
-        hp[g] = v.haps_prefix[g0 + g] - h0;
+    int64_t rbase = 0, hbase = 0;
+    int32_t rcount = 0, hcount = 0, gcount = 0;
+    rp[0] = 0;
+    hp[0] = 0;
+    for (int k = 0; k < nparts; ++k) {
+        const phmm_batch_view& v = *parts[k].v;
+        const int g0 = parts[k].g0, g1 = parts[k].g1;
+        const Span& x = sp[(size_t)k];
+        const size_t rb = (size_t)(x.rb1 - x.rb0), hb = (size_t)(x.hb1 - x.hb0);
+        std::memcpy(P + L.read_data + rbase, v.read_data + x.rb0, rb);
+        std::memcpy(P + L.bq + rbase, v.base_quals + x.rb0, rb);
+        std::memcpy(P + L.iq + rbase, v.ins_quals + x.rb0, rb);
+        std::memcpy(P + L.dq + rbase, v.del_quals + x.rb0, rb);
+        std::memcpy(P + L.gq + rbase, v.gcp_quals + x.rb0, rb);
+        std::memcpy(P + L.hap_data + hbase, v.hap_data + x.hb0, hb);
+        std::memcpy(P + L.rl + 4 * (size_t)rcount, v.read_lengths + x.r0, 4 * (size_t)(x.r1 - x.r0));
+        std::memcpy(P + L.hl + 4 * (size_t)hcount, v.hap_lengths + x.h0, 4 * (size_t)(x.h1 - x.h0));
+        for (int i = x.r0; i < x.r1; ++i)
+            ro[rcount + (i - x.r0)] = (int32_t)(rbase + v.read_offsets[i] - x.rb0);
+        for (int i = x.h0; i < x.h1; ++i)
+            ho[hcount + (i - x.h0)] = (int32_t)(hbase + v.hap_offsets[i] - x.hb0);
+        std::memcpy(P + L.nr + 4 * (size_t)gcount, v.num_reads_per_group + g0, 4 * (size_t)(g1 - g0));
+        std::memcpy(P + L.nh + 4 * (size_t)gcount, v.num_haps_per_group + g0, 4 * (size_t)(g1 - g0));
+        for (int g = g0; g < g1; ++g) {
+            rp[gcount + (g - g0) + 1] = rcount + (v.reads_prefix[g + 1] - x.r0);
+            hp[gcount + (g - g0) + 1] = hcount + (v.haps_prefix[g + 1] - x.h0);
+        }
+        rbase += (int64_t)rb;
+        hbase += (int64_t)hb;
+        rcount += x.r1 - x.r0;
+        hcount += x.h1 - x.h0;
+        gcount += g1 - g0;
     }
+    ro[nR] = (int32_t)rbytes;
+    ho[nH] = (int32_t)hbytes;
     if (!s.h2d_done) {
         if ((rc = check(hipEventCreateWithFlags(&s.h2d_done, hipEventDisableTiming), "event")) ||
             (rc = check(hipEventCreateWithFlags(&s.compute_done, hipEventDisableTiming), "event")))
@@ -657,9 +699,9 @@ int Engine::launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1)
         return rc;
     char* D = static_cast<char*>(s.dev.p);
     phmm_batch_view dv{};
-    dv.num_groups = G;
-    dv.num_reads = nR;
-    dv.num_haps = nH;
+    dv.num_groups = (int32_t)G;
+    dv.num_reads = (int32_t)nR;
+    dv.num_haps = (int32_t)nH;
     dv.read_bytes = rbytes;
     dv.hap_bytes = hbytes;
     dv.num_alignments = nA;

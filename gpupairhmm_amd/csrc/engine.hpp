@@ -116,6 +116,12 @@ struct Engine {
     // batch and enqueue copy-in, scoring and copy-out; s.compute_done fires
     // when s.pin_scores holds the scores.
     int launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1);
+    // Groups [g0, g1) of several views, concatenated in order into one slot.
+    struct ChunkPart {
+        const phmm_batch_view* v;
+        int g0, g1;
+    };
+    int launch_parts(Slot& s, const ChunkPart* parts, int nparts);
     void free_slot(Slot& s);
 };
 

@@ -12,6 +12,7 @@
 // chosen from its own read length and coefficients only, so a request's
 // scores are the same alone and in any mix.
 #include <chrono>
+#include <climits>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -47,60 +48,6 @@ int64_t request_bytes(const phmm_batch_view& v)
            24 * (int64_t)(v.num_reads + v.num_haps) + 64 * (int64_t)v.num_groups + 4096;
 }
 
-// Concatenate request batches into one batch (groups in order).
-void concat(const std::vector<const HostBatch*>& parts, HostBatch& out)
-{
-    out.clear();
-    size_t rb = 0, hb = 0, nr = 0, nh = 0, ng = 0;
-    for (const HostBatch* p : parts) {
-        rb += p->read_data.size();
-        hb += p->hap_data.size();
-        nr += p->read_lengths.size();
-        nh += p->hap_lengths.size();
-        ng += p->num_reads.size();
-    }
-    for (auto* v : {&out.read_data, &out.base_quals, &out.ins_quals, &out.del_quals, &out.gcp_quals})
-        v->reserve(rb);
-    out.hap_data.reserve(hb);
-    out.read_lengths.reserve(nr);
-    out.hap_lengths.reserve(nh);
-    out.read_offsets.reserve(nr + 1);
-    out.hap_offsets.reserve(nh + 1);
-    out.num_reads.reserve(ng);
-    out.num_haps.reserve(ng);
-    out.read_offsets.push_back(0);
-    out.hap_offsets.push_back(0);
-    out.reads_prefix.push_back(0);
-    out.haps_prefix.push_back(0);
-    auto app = [](std::vector<uint8_t>& d, const std::vector<uint8_t>& s) {
-        d.insert(d.end(), s.begin(), s.end());
-    };
-    for (const HostBatch* p : parts) {
-        const int32_t rbase = (int32_t)out.read_data.size(), hbase = (int32_t)out.hap_data.size();
-        const int32_t rcount = (int32_t)out.read_lengths.size();
-        const int32_t hcount = (int32_t)out.hap_lengths.size();
-        app(out.read_data, p->read_data);
-        app(out.base_quals, p->base_quals);
-        app(out.ins_quals, p->ins_quals);
-        app(out.del_quals, p->del_quals);
-        app(out.gcp_quals, p->gcp_quals);
-        app(out.hap_data, p->hap_data);
-        out.read_lengths.insert(out.read_lengths.end(), p->read_lengths.begin(), p->read_lengths.end());
-        out.hap_lengths.insert(out.hap_lengths.end(), p->hap_lengths.begin(), p->hap_lengths.end());
-        for (size_t i = 1; i < p->read_offsets.size(); ++i)
-            out.read_offsets.push_back(rbase + p->read_offsets[i]);
-        for (size_t i = 1; i < p->hap_offsets.size(); ++i)
-            out.hap_offsets.push_back(hbase + p->hap_offsets[i]);
-        for (size_t g = 0; g < p->num_reads.size(); ++g) {
-            out.num_reads.push_back(p->num_reads[g]);
-            out.num_haps.push_back(p->num_haps[g]);
-            out.reads_prefix.push_back(rcount + p->reads_prefix[g + 1]);
-            out.haps_prefix.push_back(hcount + p->haps_prefix[g + 1]);
-        }
-        out.num_alignments += p->num_alignments;
-    }
-}
-
 }  // namespace
 }  // namespace phmm
 
@@ -121,7 +68,6 @@ struct phmm_server {
     struct Launch {
         std::vector<Request> reqs;
         int slot = 0;
-        phmm::HostBatch merged;
     };
     std::deque<Launch> running;
     phmm::Engine::Slot slots[2];
@@ -184,8 +130,16 @@ void phmm_server::run()
                     return;
             }
             if (!queue.empty() && running.size() < 2) {
+                // merge waiting requests, up to the per-launch limits (at
+                // least one request), so the next launch can be staged on
+                // the host while this one computes
                 const int cap = opt.max_batch_requests > 0 ? opt.max_batch_requests : 1 << 30;
-                while (!queue.empty() && (int)batch.size() < cap) {
+                const int64_t acap =
+                    opt.max_batch_alignments > 0 ? opt.max_batch_alignments : INT64_MAX;
+                int64_t na = 0;
+                while (!queue.empty() && (int)batch.size() < cap &&
+                       (batch.empty() || na + queue.front().batch.num_alignments <= acap)) {
+                    na += queue.front().batch.num_alignments;
                     batch.push_back(std::move(queue.front()));
                     queue.pop_front();
                 }
@@ -195,25 +149,32 @@ void phmm_server::run()
         if (!batch.empty()) {
             Launch L;
             L.slot = slot_busy[0] ? 1 : 0;
-            std::vector<const phmm::HostBatch*> parts;
-            for (auto& r : batch)
-                parts.push_back(&r.batch);
-            phmm::concat(parts, L.merged);
             L.reqs = std::move(batch);
-            const phmm_batch_view v = L.merged.view();
+            // the requests are staged straight into the slot (no merged copy)
+            std::vector<phmm_batch_view> views;
+            std::vector<phmm::Engine::ChunkPart> parts;
+            views.reserve(L.reqs.size());
+            int64_t na = 0, cells = 0;
+            for (auto& r : L.reqs) {
+                views.push_back(r.batch.view());
+                na += r.batch.num_alignments;
+                cells += phmm_batch_cells(&views.back());
+            }
+            for (auto& v : views)
+                parts.push_back({&v, 0, v.num_groups});
             int rc = PHMM_OK;
             {
                 std::lock_guard<std::mutex> elk(*eng_mu);
                 hipSetDevice(eng->device);
-                if (v.num_alignments > 0)
-                    rc = eng->launch_chunk(slots[L.slot], v, 0, v.num_groups);
+                if (na > 0)
+                    rc = eng->launch_parts(slots[L.slot], parts.data(), (int)parts.size());
                 else
                     rc = eng->check(hipEventRecord(slots[L.slot].compute_done, eng->stream), "event");
             }
             std::unique_lock<std::mutex> lk(mu);
             st.launches += 1;
             if (rc == PHMM_OK)
-                st.cells += phmm_batch_cells(&v);
+                st.cells += cells;
             if (rc != PHMM_OK) {
                 error = rc;
                 error_msg = eng->err;
@@ -259,6 +220,7 @@ extern "C" void phmm_server_options_default(phmm_server_options* o)
     o->hbm_budget_bytes = (int64_t)64 << 30;  // 64 GiB of request state
     o->max_inflight = 1 << 20;
     o->max_batch_requests = 4096;
+    o->max_batch_alignments = 4 << 20;
 }
 
 extern "C" int phmm_server_create(phmm_engine* e, const phmm_server_options* opt,

@@ -216,7 +216,9 @@ typedef struct phmm_server_options {
     int64_t hbm_budget_bytes;  /* device bytes of admitted request state */
     int32_t max_inflight;      /* requests admitted but not yet returned */
     int32_t max_batch_requests;/* requests merged into one device launch */
-    int32_t reserved[8];
+    int64_t max_batch_alignments; /* alignments merged into one launch (a
+                                  request larger than this runs alone) */
+    int32_t reserved[6];
 } phmm_server_options;
 
 void phmm_server_options_default(phmm_server_options* opt);

@@ -205,12 +205,13 @@ def test_device_api_matches_host_api(engine):
     assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
 
 
-def test_server_alone_equals_mix(engine):
+@pytest.mark.parametrize("max_batch_alignments", [None, 1])
+def test_server_alone_equals_mix(engine, max_batch_alignments):
     import gpupairhmm_amd as G
     reqs = [G.HostBatch.synthetic(4, seed=500 + i, num_records=1) for i in range(6)]
-    say("case: serving alone vs mix")
+    say(f"case: serving alone vs mix (max_batch_alignments {max_batch_alignments})")
     alone = [engine.score_batch(r) for r in reqs]
-    srv = G.Server(engine)
+    srv = G.Server(engine, max_batch_alignments=max_batch_alignments)
     try:
         for i, r in enumerate(reqs):
             assert srv.submit(r, i)
@@ -227,6 +228,8 @@ def test_server_alone_equals_mix(engine):
     for i in range(len(reqs)):
         assert np.array_equal(alone[i].view(np.uint32), got[i].view(np.uint32))
     assert st["completed"] == len(reqs)
+    if max_batch_alignments == 1:  # every request is its own launch
+        assert st["launches"] == len(reqs)
 
 
 def test_server_budget_refusal(engine):
