@@ -38,7 +38,7 @@ __global__ void k_encode_reads(BinParams bp)
     const BatchDesc& b = *bp.bd;
     GRID_STRIDE(i, b.read_bytes)
     {
-        const uint32_t c = (uint32_t)base_code(b.read_data[i]);
+        const uint32_t c = (uint32_t)base_code_sem(b.read_data[i], bp.semantics);
         const uint32_t q = min((uint32_t)b.bq[i], 127u);
         const uint32_t qi = min((uint32_t)b.iq[i], 127u);
         const uint32_t qd = min((uint32_t)b.dq[i], 127u);
@@ -50,7 +50,7 @@ __global__ void k_encode_haps(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     GRID_STRIDE(i, b.hap_bytes)
-    bp.hapc[i] = (uint8_t)base_code(b.hap_data[i]);
+    bp.hapc[i] = (uint8_t)base_code_sem(b.hap_data[i], bp.semantics);
 }
 
 __global__ void k_hap_max(BinParams bp)
@@ -93,6 +93,17 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
             continue;
         }
         const int off = b.read_off[i];
+        if (bp.semantics == 1) {
+            bool exotic = false;
+            for (int r = lane; r < L; r += 64)
+                exotic = exotic || row_code(bp.rows[off + r]) == kExoticCode;
+            if (__any(exotic)) {
+                m.bin = 2 * bp.ntiles + 1;
+                if (lane == 0)
+                    bp.meta[i] = m;
+                continue;
+            }
+        }
         float eps = kEpsGpu;
         if (bp.semantics == 1)
             eps = bp.ph2pr[min((int)b.gcp[off], 127)];
@@ -177,8 +188,8 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
 __global__ __launch_bounds__(256) void k_group_bins(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    __shared__ int cnt[2 * kMaxTiles + 2];
-    __shared__ int pos[2 * kMaxTiles + 2];
+    __shared__ int cnt[kMaxBins + 1];
+    __shared__ int pos[kMaxBins + 1];
     const int NB1 = bp.NB + 1;  // slot NB: empty reads
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
         for (int k = threadIdx.x; k < NB1; k += blockDim.x)
@@ -214,6 +225,32 @@ __device__ __forceinline__ int key_of(int bin, int hl)
     return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
 }
 
+// Reference CPU semantics: per hap (one wave each), whether it holds a byte
+// other than A/C/G/T/N.
+__global__ __launch_bounds__(256) void k_hap_flags(BinParams bp)
+{
+    const BatchDesc& b = *bp.bd;
+    const int lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (int h = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); h < b.nH; h += nwaves) {
+        const int hl = b.hap_len[h], off = b.hap_off[h];
+        bool exotic = false;
+        for (int c = lane; c < hl; c += 64)
+            exotic = exotic || bp.hapc[off + c] == kExoticCode;
+        exotic = __any(exotic);
+        if (lane == 0)
+            bp.hapflag[h] = exotic ? 1 : 0;
+    }
+}
+
+// The bin an alignment of read bin k and hap h goes to: a hap with bytes
+// other than A/C/G/T/N (CPU semantics) sends its tile-bin alignments to the
+// exact bin.
+__device__ __forceinline__ int bin_of(const BinParams& bp, int k, int h)
+{
+    return (bp.semantics == 1 && k < 2 * bp.ntiles && bp.hapflag[h]) ? 2 * bp.ntiles + 1 : k;
+}
+
 // Per group: add its alignment counts to the (bin, hap length) histogram.
 __global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
 {
@@ -228,7 +265,7 @@ __global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
             for (int k = 0; k < bp.NB; ++k) {
                 const int n = bp.gb_count[(int64_t)g * NB1 + k];
                 if (n > 0)
-                    atomicAdd(&bp.key_count[key_of(k, hl)], n);
+                    atomicAdd(&bp.key_count[key_of(bin_of(bp, k, h0 + h), hl)], n);
             }
         }
     }
@@ -331,7 +368,8 @@ __global__ __launch_bounds__(256) void k_scatter(BinParams bp)
                     }
                     continue;
                 }
-                const int base = bp.bin_range[2 * k] + atomicAdd(&bp.key_cursor[key_of(k, hl)], n);
+                const int kb = bin_of(bp, k, h0 + h);
+                const int base = bp.bin_range[2 * kb] + atomicAdd(&bp.key_cursor[key_of(kb, hl)], n);
                 for (int j = 0; j < n; ++j) {
                     const int rd = bp.read_perm[r0 + st + j];
                     WorkItem it;
@@ -389,6 +427,8 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.nR > 0)  // one wave per read
         hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
+    if (bp.semantics == 1 && (fixed_grids || hd.nH > 0))  // one wave per hap
+        hipLaunchKernelGGL(k_hap_flags, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
         hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);

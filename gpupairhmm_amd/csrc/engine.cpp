@@ -61,7 +61,8 @@ Engine::~Engine()
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
     for (DevBuf* b : {&rows, &hapc, &meta, &gb_count, &gb_start, &read_perm, &align_off, &key_count,
-                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &final_list, &spill})
+                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &final_list, &spill,
+                      &hapflag})
         if (b->p)
             hipFree(b->p);
     for (auto& s : slots)
@@ -185,7 +186,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (v.num_haps > 0 && !v.hap_data))
         return set_error(PHMM_EINVAL, "score_device: null device pointer in batch view");
     const int ntiles = (int)tiles.size();
-    const int NB = 2 * ntiles + 1;
+    const int NB = 2 * ntiles + 2;  // + multi-tile bin + exact bin
     const int G = v.num_groups;
     const int64_t nA = v.num_alignments;
     int rc;
@@ -203,7 +204,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
         (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")) ||
         (rc = ensure(final_list, sizeof(int32_t) * (size_t)nA, "final list")) ||
-        (rc = ensure(desc, sizeof(BatchDesc), "batch descriptor")))
+        (rc = ensure(desc, sizeof(BatchDesc), "batch descriptor")) ||
+        (rc = ensure(hapflag, (size_t)std::max(v.num_haps, 1), "hap flags")))
         return rc;
     const size_t spill_bytes = (size_t)256 << 20;
     if ((rc = ensure(spill, spill_bytes, "spill")))
@@ -281,6 +283,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.error_flag = sm + 2;
     bp.rescue_counts = sm + 4;
     bp.task_ctr = sm + 128;
+    bp.hapflag = static_cast<uint8_t*>(hapflag.p);
     bp.items = static_cast<WorkItem*>(items.p);
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
@@ -436,20 +439,24 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                 return rc;
         }
     }
+    // the multi-tile bin and the exact bin: one wave per alignment
+    const int eb = mb + 1;
     if (plain) {
-        const int n = h_bin_range[2 * mb + 1];
+        const int n = h_bin_range[2 * mb + 1] + h_bin_range[2 * eb + 1];
         if (n <= 0)
             return PHMM_OK;
         stats.multitile_alignments += n;
     }
     BinSet ws;
-    ws.n = 1;
+    ws.n = 2;
     ws.bin[0] = mb;
-    ws.lanes[0] = 64;
+    ws.bin[1] = eb;
+    ws.lanes[0] = ws.lanes[1] = 64;
     ws.range = bp.bin_range;
     ws.counts = bp.bin_range + 1;
     ws.count_stride = 2;
-    const int blocks = plain ? std::min(num_cus * 8, h_bin_range[2 * mb + 1]) : num_cus * 8;
+    const int blocks =
+        plain ? std::min(num_cus * 8, h_bin_range[2 * mb + 1] + h_bin_range[2 * eb + 1]) : num_cus * 8;
     return check(launch_fwd_wave_f32(fp, ws, bp.hmax, spill32, n32, bp.error_flag, blocks, st),
                  "multi-tile kernel");
 }
@@ -891,7 +898,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
         (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipHostMalloc(&e.h_bin_range, 2 * sizeof(int32_t) * (2 * phmm::kMaxTiles + 1),
+        (rc = e.check(hipHostMalloc(&e.h_bin_range, 2 * sizeof(int32_t) * phmm::kMaxBins,
                                     hipHostMallocDefault),
                       "pinned")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")))

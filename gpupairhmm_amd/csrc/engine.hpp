@@ -38,7 +38,7 @@ struct Engine {
 
     // scratch (grow-only)
     DevBuf rows, hapc, meta, gb_count, gb_start, read_perm, align_off, key_count, key_cursor,
-        bin_range, smallints, items, rescue_list, final_list, spill;
+        bin_range, smallints, items, rescue_list, final_list, spill, hapflag;
     int32_t* h_bin_range = nullptr;   // pinned, 2*NB
     int32_t* h_small = nullptr;       // pinned, small readbacks
     // event pairs around the forward kernels of timed calls

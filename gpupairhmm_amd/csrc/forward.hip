@@ -50,8 +50,9 @@ namespace phmm {
 
 __device__ __forceinline__ int shr1i(int v)
 {
-    // DPP wave_shr:1 -- lane i receives lane i-1; lane 0 receives 0.
-    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+    // DPP wave_shr:1 -- lane i receives lane i-1; lane 0 receives 0 (bound
+    // control: an out-of-range source reads as zero).
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float shr1f(float v)
 {
@@ -79,6 +80,12 @@ __device__ __forceinline__ int wave_min(int v)
         v = min(v, __shfl_xor(v, o, 64));
     return __builtin_amdgcn_readfirstlane(v);
 }
+
+// Setup-time division: FP32 coefficients use the hardware reciprocal (a few
+// ulp; the recurrence's result is unaffected at its tolerance), FP64 keeps
+// the correctly rounded quotient.
+__device__ __forceinline__ float qdiv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
 
 __device__ __forceinline__ float ph2pr_of(const float* table, int q)
 {
@@ -188,16 +195,16 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 const T alpha = (T)1 - (delta + sigma);
                 code[j] = row_code(w);
                 lm[j] = (T)1 - pb;
-                lx[j] = pb / (T)3;
+                lx[j] = qdiv(pb, (T)3);
                 if (FAST) {
                     if (ri == 0) {
                         c0[j] = 0;
-                        c1[j] = beta / alpha;
+                        c1[j] = qdiv(beta, alpha);
                         c2[j] = 0;
                     } else {
-                        c0[j] = beta * pdelta / (palpha * alpha);
-                        c1[j] = beta * psigma / alpha;
-                        c2[j] = eps * pdelta / (delta * palpha);
+                        c0[j] = qdiv(beta * pdelta, palpha * alpha);
+                        c1[j] = qdiv(beta * psigma, alpha);
+                        c2[j] = qdiv(eps * pdelta, delta * palpha);
                     }
                     c3[j] = 0;
                 } else {
@@ -217,7 +224,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 if (FAST) {
                     c0[j] = 0;
                     c1[j] = 0;
-                    c2[j] = firstoob ? pdelta / palpha : (T)1;
+                    c2[j] = firstoob ? qdiv(pdelta, palpha) : (T)1;
                     c3[j] = 0;
                 } else {
                     c0[j] = 0;
@@ -263,18 +270,27 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
     // 0/1 lane mask, which issues at the full FP32 rate where a select does not
     const T nf = first ? (T)0 : (T)1;
     const T c0f = first ? C0 : (T)0;
-    T Ml = 0, Dl = c0f;
-    T Mul = 0, Dul = Dl;
+    // M and D of the row above reach a lane two steps after they were
+    // computed (diagonal dependency), I one step after: step u reads ring
+    // entry u % 2 and refills it, so the unrolled steps need no copies
+    T Mr[2] = {(T)0, (T)0}, Dr[2] = {c0f, c0f};
     T Iin = 0;
     T acc = 0;
-    int letter = 4;
 
     const int Hmax = wave_max(HL);
     const int nsteps = Hmax + G;
-    const uint32_t* hdw = reinterpret_cast<const uint32_t*>(p.hapc + hoff);
-    const int hdw_n = active ? (HL >> 2) + 1 : 0;  // padded length / 4
-    uint32_t wcur = (first && 0 < hdw_n) ? hdw[0] : 0x04040404u;
-    uint32_t wnext = (first && 1 < hdw_n) ? hdw[1] : 0x04040404u;
+    // Hap letters: lane t consumes column c = s - t at step s.  Each lane
+    // keeps a window of two dwords of its hap codes and shifts it by
+    // (c mod 4) bytes; dword indices are clamped to the padded hap, so every
+    // column up to the padding is exact, and columns outside it (before the
+    // lane's first column or past the hap's end) only meet zero state or are
+    // never accumulated.
+    const uint32_t* hdw = reinterpret_cast<const uint32_t*>(p.hapc + (active ? hoff : 0));
+    const int hdw_last = active ? (HL >> 2) : 0;  // padded length / 4 - 1
+    const uint32_t shift = (uint32_t)(-t) & 3u;     // (s4 - t) mod 4, constant per lane
+    auto hap_dword = [&](int q) { return hdw[min(max(q, 0), hdw_last)]; };
+    const int q0 = (0 - t) >> 2;  // floor((s4 - t) / 4) of the first block
+    uint32_t wlo = hap_dword(q0), whi = hap_dword(q0 + 1), wnx = hap_dword(q0 + 2);
     // Step s adds the chain of column s - t.  The chain is exactly 0 before
     // a lane's first column, so only the end of the lane's real columns,
     // s <= t + HL, needs a bound: steps below the wave's smallest HL add
@@ -283,40 +299,38 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
     const int n_free = wave_min(active ? HL : nsteps) + 1;  // steps s < n_free
     const int lane_off = lane;  // tab index of this lane's letter-0 entry
 
-    // letter and emission values of step 0; each step loads the next step's
-    // values before it computes (the next letter is known one step ahead)
-    letter = first ? (int)(wcur & 0xffu) : 4;
-    V lamv[NQ];
+    // emission values of step 0; each step loads the next step's values
+    // before it computes
+    V lam[2][NQ];
+    uint32_t letters = __builtin_amdgcn_alignbyte(whi, wlo, shift);
     {
-        const V* trow = tab + __umul24((unsigned)letter, NQ * 64) + lane_off;
+        const V* trow = tab + __umul24(letters & 0xffu, NQ * 64) + lane_off;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            lamv[q] = trow[q * 64];
+        for (int qq = 0; qq < NQ; ++qq)
+            lam[0][qq] = trow[qq * 64];
     }
 
     auto sweep4 = [&](int s4, bool bounded) {
+        // letters of steps s4+1 .. s4+4: the current window and the next one
+        const uint32_t next = __builtin_amdgcn_alignbyte(wnx, whi, shift);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            // next step's letter: the group's first lane reads the hap, the
-            // others take their left neighbour's current letter
-            const int sh = shr1i(letter);
-            const uint32_t wsrc = (u == 3) ? wnext : wcur;
-            const int nletter = first ? (int)((wsrc >> (8 * ((u + 1) & 3))) & 0xffu) : sh;
-            V lamn[NQ];
+            const int cur = u & 1, nxt = cur ^ 1;
+            const uint32_t nletter = u < 3 ? (letters >> (8 * (u + 1))) & 0xffu : next & 0xffu;
             {
-                const V* trow = tab + __umul24((unsigned)nletter, NQ * 64) + lane_off;
+                const V* trow = tab + __umul24(nletter, NQ * 64) + lane_off;
 #pragma unroll
-                for (int q = 0; q < NQ; ++q)
-                    lamn[q] = trow[q * 64];
+                for (int qq = 0; qq < NQ; ++qq)
+                    lam[nxt][qq] = trow[qq * 64];
             }
-            T mp = Mul, dp = Dul, chain = Iin;
+            T mp = Mr[cur], dp = Dr[cur], chain = Iin;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                T lam[K];
-                TT::unpack(lamv[q], lam);
+            for (int qq = 0; qq < NQ; ++qq) {
+                T lv[K];
+                TT::unpack(lam[cur][qq], lv);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const int j = K * q + k;
+                    const int j = K * qq + k;
                     const T mo = M[j], dov = D[j];
                     T t1;
                     if (FAST) {
@@ -328,7 +342,7 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                         t1 = fma(c0[j], mp, t1);
                         chain = fma(c2[j], mp, c3[j] * chain);
                     }
-                    M[j] = lam[k] * t1;
+                    M[j] = lv[k] * t1;
                     D[j] = fma(eps, dov, mo);
                     mp = mo;
                     dp = dov;
@@ -338,22 +352,14 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 acc += (s4 + u <= acc_last) ? chain : (T)0;
             else
                 acc += chain;
-            const T sM = TT::shr(M[NR - 1]);
-            const T sD = TT::shr(D[NR - 1]);
-            const T sI = TT::shr(chain);
-            Mul = Ml;
-            Dul = Dl;
-            Ml = sM * nf;
-            Dl = fma(sD, nf, c0f);
-            Iin = sI * nf;
-            letter = nletter;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                lamv[q] = lamn[q];
+            Mr[cur] = TT::shr(M[NR - 1]) * nf;
+            Dr[cur] = fma(TT::shr(D[NR - 1]), nf, c0f);
+            Iin = TT::shr(chain) * nf;
         }
-        const int k = (s4 >> 2) + 2;
-        wcur = wnext;
-        wnext = (first && k < hdw_n) ? hdw[k] : 0x04040404u;
+        letters = next;
+        wlo = whi;
+        whi = wnx;
+        wnx = hap_dword(((s4 - t) >> 2) + 3);  // arithmetic shift: floor
     };
     int s4 = 0;
     for (; s4 + 4 <= n_free && s4 < nsteps; s4 += 4)
@@ -380,7 +386,11 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
         } else {
             const double a = (double)acc;
             bd.scores[out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
-            if (!(a > 0.0 && a < 1e300)) {
+            // one scale for the whole alignment: accept sums that stay far
+            // above double's floor (score above ~ -572), where no cell that
+            // matters can have been flushed; the rest go to the
+            // exponent-tracking wave kernel
+            if (!(a >= 0x1p-900 && a < 1e300)) {
                 const int n = atomicAdd(p.final_count, 1);
                 p.final_list[n] = idx;
             }
@@ -572,6 +582,11 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
         }
         const T C0 = S::ldexp((T)1, 64) / (T)HL;
         const int ntiles = (L + 1 + TR - 1) / TR;
+        // Emission match: codes with 4 as the wildcard (reference GPU
+        // semantics), or the raw bytes with a literal 'N' as the wildcard
+        // (reference CPU semantics, gpuworker.cuh:1063) -- exact for any byte.
+        const bool raw = p.semantics == 1;
+        const int wild = raw ? (int)'N' : 4;
         T acc = 0;
         int sc = 0;
         for (int k = 0; k < ntiles; ++k) {
@@ -591,8 +606,8 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
                     cs[j] = si;
                     ce[j] = eps;
                     lm[j] = (T)1 - pb;
-                    code[j] = row_code(w32);
-                    lx[j] = code[j] == 4 ? lm[j] : pb / (T)3;
+                    code[j] = raw ? (int)bd.read_data[roff + ri] : row_code(w32);
+                    lx[j] = code[j] == wild ? lm[j] : pb / (T)3;
                 } else {
                     ca[j] = 0;
                     cdl[j] = ri == L ? (T)1 : (T)0;
@@ -600,7 +615,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
                     ce[j] = 1;
                     lm[j] = 0;
                     lx[j] = 0;
-                    code[j] = 4;
+                    code[j] = wild;
                 }
             }
             T M[NR], D[NR];
@@ -646,7 +661,9 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
             auto expo = [&](T v, int e) { return v > (T)0 ? S::ilogb(v) + e : kNone; };
             for (int s = 0; s < nsteps; ++s) {
                 const int c = s - lane;  // this lane's 0-based column
-                const int letter = (c >= 0 && c < HL) ? (int)p.hapc[hoff + c] : 4;
+                const int letter = (c >= 0 && c < HL)
+                                       ? (raw ? (int)bd.hap_data[hoff + c] : (int)p.hapc[hoff + c])
+                                       : wild;
                 T mp = Mul, dp = Dul, chain = Iin;
                 if (lane == 0 && !top) {
                     // row above the tile at column c-1, from the previous tile
@@ -671,7 +688,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
                     const T mo = M[j], dov = D[j];
-                    const T lam = (letter == code[j] || letter == 4) ? lm[j] : lx[j];
+                    const T lam = (letter == code[j] || letter == wild) ? lm[j] : lx[j];
                     M[j] = lam * (ca[j] * mp + beta * (chain + dp));
                     chain = cdl[j] * mp + ce[j] * chain;
                     D[j] = cs[j] * mo + eps * dov;

@@ -33,6 +33,9 @@ constexpr double kStartF64 = 0x1p1000;
 constexpr double kLog10StartF64 = 301.02999566398120;  // 1000 * log10(2)
 
 constexpr int kMaxTiles = 32;
+// Bins: 2 per tile class (FAST, SAFE), the multi-tile bin, and the exact
+// bin (reference CPU semantics, bytes other than A/C/G/T/N present).
+constexpr int kMaxBins = 2 * kMaxTiles + 2;
 constexpr int kTaskCounters = 32;   // set launches per call
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 
@@ -61,6 +64,16 @@ __host__ __device__ inline int row_dq(uint32_t w) { return (int)(w >> 24); }
 __host__ __device__ inline int base_code(uint8_t b)
 {
     return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : 4;
+}
+
+// Reference CPU semantics (align_host, gpuworker.cuh:1063): only a literal
+// 'N' is the wildcard; any byte other than A/C/G/T/N gets code 5 and sends
+// its read or hap to the exact bin, whose kernel compares raw bytes.
+constexpr int kExoticCode = 5;
+__host__ __device__ inline int base_code_sem(uint8_t b, int semantics)
+{
+    const int c = base_code(b);
+    return (semantics == 1 && c == 4 && b != 'N') ? kExoticCode : c;
 }
 
 // Per-read metadata from the read pass.
@@ -131,6 +144,7 @@ struct BinParams {
     int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
     int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
+    uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T/N (CPU semantics)
     WorkItem* items;
     const float* ph2pr;
     // tiles
@@ -166,8 +180,8 @@ hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaun
 // *next_task (zeroed before the launch).
 struct BinSet {
     int32_t n = 0;
-    int32_t bin[2 * kMaxTiles + 1] = {};
-    int32_t lanes[2 * kMaxTiles + 1] = {};  // tile kernels: lanes per alignment
+    int32_t bin[kMaxBins] = {};
+    int32_t lanes[kMaxBins] = {};  // tile kernels: lanes per alignment
     const int32_t* list = nullptr;
     const int32_t* range = nullptr;
     const int32_t* counts = nullptr;

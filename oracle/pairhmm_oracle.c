@@ -45,6 +45,58 @@ static int is_match(uint8_t x, uint8_t h, int semantics)
 
 static inline int qclamp(uint8_t q) { return q > 127 ? 127 : q; }
 
+/* A double with its own exponent epoch: value = m * 2^(512*e), with m == 0
+ * or 2^-512 <= m < 2^512.  The DP values of one row can span far more than
+ * double's range (a decaying D carry-over next to a path that later
+ * dominates, at scores below about -300), so every cell carries its epoch;
+ * neighbouring cells almost always share it, which keeps the arithmetic
+ * plain double. */
+typedef struct {
+    double m;
+    int e;
+} xd;
+
+static inline xd xd_norm(xd x)
+{
+    /* magnitudes: qualities of 0 make alpha negative and cells signed (as
+     * in the reference); non-finite values are left alone */
+    if (!isfinite(x.m))
+        return x;
+    while (fabs(x.m) >= 0x1p512) {
+        x.m *= 0x1p-512;
+        x.e += 1;
+    }
+    while (x.m != 0.0 && fabs(x.m) < 0x1p-512) {
+        x.m *= 0x1p512;
+        x.e -= 1;
+    }
+    return x;
+}
+
+static inline xd xd_scale(xd x, double k)
+{
+    x.m *= k;
+    return x;  /* k <= 1 and >= ~1e-13: renormalized by the caller */
+}
+
+static inline xd xd_add(xd a, xd b)
+{
+    if (a.m == 0.0)
+        return b;
+    if (b.m == 0.0)
+        return a;
+    if (a.e < b.e) {
+        const xd t = a;
+        a = b;
+        b = t;
+    }
+    const int d = a.e - b.e;
+    if (d >= 3)  /* b < 2^-512 of a */
+        return a;
+    a.m += ldexp(b.m, -512 * d);
+    return a;
+}
+
 static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
                         const uint8_t* bq, const uint8_t* iq, const uint8_t* dq,
                         uint8_t gcp0, int semantics, const float* P)
@@ -58,50 +110,40 @@ static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
         eps = (double)0.1f;
         beta = (double)0.9f;
     }
-    double* buf = (double*)malloc(sizeof(double) * 6 * (size_t)(H + 1));
-    double *Mp = buf, *Ip = buf + (H + 1), *Dp = buf + 2 * (H + 1);
-    double *Mc = buf + 3 * (H + 1), *Ic = buf + 4 * (H + 1), *Dc = buf + 5 * (H + 1);
+    xd* buf = (xd*)malloc(sizeof(xd) * 6 * (size_t)(H + 1));
+    xd *Mp = buf, *Ip = buf + (H + 1), *Dp = buf + 2 * (H + 1);
+    xd *Mc = buf + 3 * (H + 1), *Ic = buf + 4 * (H + 1), *Dc = buf + 5 * (H + 1);
+    const xd zero = {0.0, 0};
     for (int c = 0; c <= H; ++c) {
-        Mp[c] = 0.0;
-        Ip[c] = 0.0;
-        Dp[c] = constant / H;
+        Mp[c] = zero;
+        Ip[c] = zero;
+        Dp[c] = xd_norm((xd){constant / H, 0});
     }
-    /* Power-of-two row scaling keeps FP64 out of underflow for scores far
-     * below its range (log10 < -300): a row whose largest value drops under
-     * 2^-600 is multiplied by 2^600 (exact) and the exponent is carried. */
-    long exp2 = 0;
     for (int r = 1; r <= R; ++r) {
         const double p = P[qclamp(bq[r - 1])];
         const double delta = P[qclamp(iq[r - 1])];
         const double sigma = P[qclamp(dq[r - 1])];
         const double alpha = 1.0 - (delta + sigma);
-        Mc[0] = Ic[0] = Dc[0] = 0.0;
-        double mx = 0.0;
+        Mc[0] = Ic[0] = Dc[0] = zero;
         for (int c = 1; c <= H; ++c) {
             const double lambda = is_match(read[r - 1], hap[c - 1], semantics) ? 1.0 - p : p / 3.0;
-            Mc[c] = lambda * (alpha * Mp[c - 1] + beta * (Ip[c - 1] + Dp[c - 1]));
-            Ic[c] = delta * Mp[c] + eps * Ip[c];
-            Dc[c] = sigma * Mc[c - 1] + eps * Dc[c - 1];
-            mx = fmax(mx, fmax(Mc[c], fmax(Ic[c], Dc[c])));
+            const xd gap = xd_add(Ip[c - 1], Dp[c - 1]);
+            Mc[c] = xd_norm(xd_scale(xd_add(xd_scale(Mp[c - 1], alpha), xd_scale(gap, beta)), lambda));
+            Ic[c] = xd_norm(xd_add(xd_scale(Mp[c], delta), xd_scale(Ip[c], eps)));
+            Dc[c] = xd_norm(xd_add(xd_scale(Mc[c - 1], sigma), xd_scale(Dc[c - 1], eps)));
         }
-        if (mx > 0.0 && mx < 0x1p-600) {
-            for (int c = 1; c <= H; ++c) {
-                Mc[c] = ldexp(Mc[c], 600);
-                Ic[c] = ldexp(Ic[c], 600);
-                Dc[c] = ldexp(Dc[c], 600);
-            }
-            exp2 -= 600;
-        }
-        double* t;
+        xd* t;
         t = Mp; Mp = Mc; Mc = t;
         t = Ip; Ip = Ic; Ic = t;
         t = Dp; Dp = Dc; Dc = t;
     }
-    double sum = 0.0;
+    xd sum = zero;
     for (int c = 1; c <= H; ++c)
-        sum += Mp[c] + Ip[c];
+        sum = xd_norm(xd_add(sum, xd_norm(xd_add(Mp[c], Ip[c]))));
     free(buf);
-    return log10(sum) + (double)exp2 * 0.30102999566398119521 - log10(constant);
+    if (sum.m == 0.0)
+        return -INFINITY;
+    return log10(sum.m) + (double)sum.e * 512.0 * 0.30102999566398119521 - log10(constant);
 }
 
 /* Float cells, double result: the rounding of align_host, step for step. */

@@ -8,7 +8,9 @@
  * It restates the reference's CPU forward algorithm `align_host`
  * (gpuworker.cuh:1022-1098) and its batch driver `processBatchCPU`
  * (gpuworker.cuh:975-1020) in plain C, in two arithmetic flavours:
- *   - ORACLE_F64      : every quantity in double (the FP64 oracle);
+ *   - ORACLE_F64      : every quantity in double, each cell with its own
+ *                       power-of-two exponent epoch (the FP64 oracle; exact
+ *                       at any score, whatever range a row spans);
  *   - ORACLE_F32CELLS : float M/I/D cells and coefficients, double result
  *                       sum -- the exact rounding of align_host.
  * and two input semantics:

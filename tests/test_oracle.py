@@ -12,8 +12,8 @@ swapped I/D, a wrong constant) fails at least one:
   * closed forms for R=1 and for H=1;
   * the phred table against 10^(-q/10) and against the reference's
     generate_ph2pr;
-  * exact power-of-two row scaling (extreme scores) against an
-    arbitrary-exponent Decimal evaluation.
+  * per-cell exponent epochs (extreme scores, rows wider than double's
+    range) against an arbitrary-exponent Decimal evaluation.
 """
 import itertools
 import math
@@ -75,6 +75,19 @@ def test_float_cells_match_reference_align_host_bit_exact(oracle):
         a = oracle.align(hap, read, bq, iq, dq, gcp, oracle.F32CELLS, oracle.SEM_CPU)
         b = oracle.ref_align(hap, read, bq, iq, dq, gcp)
         assert (a == b) or (math.isinf(a) and math.isinf(b)), (a, b)
+    # bytes other than A/C/G/T/N in reads and haps: align_host compares raw
+    # bytes with a literal 'N' wildcard (gpuworker.cuh:1063)
+    for _ in range(100):
+        R, H = int(rng.integers(1, 50)), int(rng.integers(1, 60))
+        read = "".join(rng.choice(list("ACGTNRYacgn*"), R))
+        hap = "".join(rng.choice(list("ACGTNRYacgn*"), H))
+        bq = rng.integers(5, 41, R)
+        iq = rng.integers(10, 46, R)
+        dq = rng.integers(10, 46, R)
+        gcp = int(rng.integers(5, 40))
+        a = oracle.align(hap, read, bq, iq, dq, gcp, oracle.F32CELLS, oracle.SEM_CPU)
+        b = oracle.ref_align(hap, read, bq, iq, dq, gcp)
+        assert (a == b) or (math.isinf(a) and math.isinf(b)), (read, hap, a, b)
     # the reference's own phred table
     import ctypes
     t = np.zeros(128, dtype=np.float32)
@@ -184,29 +197,19 @@ def test_empty_inputs(oracle):
     assert oracle.align("ACGT", "", [], [], []) == float("-inf")
 
 
-def test_extreme_scores_match_arbitrary_exponent_evaluation(oracle):
-    """Scores far below double's range: the oracle's exact power-of-two row
-    scaling agrees with a Decimal evaluation (unbounded exponent)."""
+def _decimal_forward(read, hap, bq, iq, dq, eps, beta):
+    """The forward algorithm in Decimal (unbounded exponent), reference
+    constants and GPU-kernel wildcard rule for ACGT input."""
     getcontext().prec = 40
-    rng = np.random.default_rng(21)
-    # a read far longer than the hap: every path inserts > 1000 bases at
-    # eps = 0.1 each, so log10 < -1000 (beyond double's range of ~ -308)
-    R, H = 1100, 30
-    read = "".join(rng.choice(list("ACGT"), R))
-    hap = "".join(rng.choice(list("ACGT"), H))
-    bq = np.full(R, 93)
-    iq = np.full(R, 93)
-    dq = np.full(R, 93)
-    got = oracle.align(hap, read, bq, iq, dq, 10, oracle.F64)
-    assert got < -1000
     P = [Decimal(float(np.float32(10.0 ** (-q / 10.0)))) for q in range(128)]
-    eps, beta = Decimal(F32_EPS), Decimal(F32_BETA)
+    eps, beta = Decimal(eps), Decimal(beta)
     C = Decimal(float(np.float32(np.finfo(np.float32).max) / np.float32(16)))
+    R, H = len(read), len(hap)
     Mp = [Decimal(0)] * (H + 1)
     Ip = [Decimal(0)] * (H + 1)
     Dp = [C / H] * (H + 1)
     for r in range(1, R + 1):
-        p, de, si = P[bq[r - 1]], P[iq[r - 1]], P[dq[r - 1]]
+        p, de, si = P[int(bq[r - 1])], P[int(iq[r - 1])], P[int(dq[r - 1])]
         al = 1 - (de + si)
         Mc, Ic, Dc = [Decimal(0)] * (H + 1), [Decimal(0)] * (H + 1), [Decimal(0)] * (H + 1)
         for c in range(1, H + 1):
@@ -216,7 +219,42 @@ def test_extreme_scores_match_arbitrary_exponent_evaluation(oracle):
             Dc[c] = si * Mc[c - 1] + eps * Dc[c - 1]
         Mp, Ip, Dp = Mc, Ic, Dc
     s = sum(Mp[c] + Ip[c] for c in range(1, H + 1))
-    want = float(s.log10() - C.log10())
+    return float(s.log10() - C.log10())
+
+
+def test_extreme_scores_match_arbitrary_exponent_evaluation(oracle):
+    """Scores far below double's range: the oracle (a per-cell exponent
+    epoch) agrees with a Decimal evaluation (unbounded exponent)."""
+    rng = np.random.default_rng(21)
+    # a read far longer than the hap: every path inserts > 1000 bases at
+    # eps = 0.1 each, so log10 < -1000 (beyond double's range of ~ -308)
+    R, H = 1100, 30
+    read = "".join(rng.choice(list("ACGT"), R))
+    hap = "".join(rng.choice(list("ACGT"), H))
+    q = np.full(R, 93)
+    got = oracle.align(hap, read, q, q, q, 10, oracle.F64)
+    assert got < -1000
+    want = _decimal_forward(read, hap, q, q, q, F32_EPS, F32_BETA)
+    assert abs(got - want) < 1e-9 * abs(want)
+
+
+def test_wide_row_range_matches_arbitrary_exponent_evaluation(oracle):
+    """A row whose values span more than double's whole range: a decaying
+    deletion carry-over at the left next to the path that dominates later
+    columns.  One scale per row (or per wave lane) would flush the path's
+    cells; scaling per row lost 11 decades here (-1618.73 for -1607.65)."""
+    rng = np.random.default_rng(0)
+    R, H = 600, 150
+    read = "".join(rng.choice(list("ACGT"), R))
+    hap = "".join(rng.choice(list("ACGT"), H))
+    bq = rng.integers(10, 41, R)
+    iq = rng.integers(25, 46, R)
+    dq = rng.integers(25, 46, R)
+    gcp = 38  # CPU semantics: eps = ph2pr[38], beta = 1 - eps
+    got = oracle.align(hap, read, bq, iq, dq, gcp, oracle.F64, oracle.SEM_CPU)
+    eps = float(np.float32(10.0 ** (-gcp / 10.0)))
+    want = _decimal_forward(read, hap, bq, iq, dq, eps, 1.0 - eps)
+    assert abs(got - (-1607.6463)) < 1e-3
     assert abs(got - want) < 1e-9 * abs(want)
 
 
