@@ -33,24 +33,47 @@ __global__ void k_clear(int32_t* p, int64_t n)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)(n); \
          i += (int64_t)gridDim.x * blockDim.x)
 
+// Sequence bytes are padded to multiples of 4 per read/hap (the reference's
+// padding rule), so both encoders work on 4 bytes per thread: dword loads of
+// each input array, one 16-byte (reads) or 4-byte (haps) store.
 __global__ void k_encode_reads(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    GRID_STRIDE(i, b.read_bytes)
+    const uint32_t* rd = reinterpret_cast<const uint32_t*>(b.read_data);
+    const uint32_t* q0 = reinterpret_cast<const uint32_t*>(b.bq);
+    const uint32_t* q1 = reinterpret_cast<const uint32_t*>(b.iq);
+    const uint32_t* q2 = reinterpret_cast<const uint32_t*>(b.dq);
+    uint4* out = reinterpret_cast<uint4*>(bp.rows);
+    GRID_STRIDE(i, b.read_bytes >> 2)
     {
-        const uint32_t c = (uint32_t)base_code_sem(b.read_data[i], bp.semantics);
-        const uint32_t q = min((uint32_t)b.bq[i], 127u);
-        const uint32_t qi = min((uint32_t)b.iq[i], 127u);
-        const uint32_t qd = min((uint32_t)b.dq[i], 127u);
-        bp.rows[i] = c | (q << 8) | (qi << 16) | (qd << 24);
+        const uint32_t r = rd[i], a = q0[i], c = q1[i], d = q2[i];
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t code = (uint32_t)base_code_sem((uint8_t)(r >> (8 * k)), bp.semantics);
+            const uint32_t qb = min((a >> (8 * k)) & 0xffu, 127u);
+            const uint32_t qi = min((c >> (8 * k)) & 0xffu, 127u);
+            const uint32_t qd = min((d >> (8 * k)) & 0xffu, 127u);
+            w[k] = code | (qb << 8) | (qi << 16) | (qd << 24);
+        }
+        out[i] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
 __global__ void k_encode_haps(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    GRID_STRIDE(i, b.hap_bytes)
-    bp.hapc[i] = (uint8_t)base_code_sem(b.hap_data[i], bp.semantics);
+    const uint32_t* hd = reinterpret_cast<const uint32_t*>(b.hap_data);
+    uint32_t* out = reinterpret_cast<uint32_t*>(bp.hapc);
+    GRID_STRIDE(i, b.hap_bytes >> 2)
+    {
+        const uint32_t h = hd[i];
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w |= (uint32_t)base_code_sem((uint8_t)(h >> (8 * k)), bp.semantics) << (8 * k);
+        out[i] = w;
+    }
 }
 
 __global__ void k_hap_max(BinParams bp)
@@ -65,52 +88,42 @@ __global__ void k_hap_max(BinParams bp)
         atomicMax(bp.hmax, m);
 }
 
-// Per read (one wave each): its bin.  Tile class k is the first table row
-// whose max_read_len covers the read (its tile has room for one more row);
-// reads beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k uses
-// the rescaled (FAST) arithmetic when the read's coefficients keep it in
-// range, else bin 2k+1 (SAFE).  The lanes check rows lane, lane+64, ... and
-// the wave combines their verdicts and the product of alpha.
+// Per read (16 lanes each, four reads per wave): its bin.  Tile class k is
+// the first table row whose max_read_len covers the read (its tile has room
+// for one more row); reads beyond the last row go to the multi-tile bin
+// 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when the read's
+// coefficients keep it in range, else bin 2k+1 (SAFE).  Reference CPU
+// semantics: a read holding a byte other than A/C/G/T/N goes to the exact
+// bin 2*ntiles+1.  The 16 lanes check rows sl, sl+16, ... and combine their
+// verdicts and the product of alpha with row-local reductions.
 __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    const int lane = threadIdx.x & 63;
-    const int nwaves = gridDim.x * (blockDim.x >> 6);
-    for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < b.nR; i += nwaves) {
-        const int L = b.read_len[i];
-        ReadMeta m;
-        m.nl = 1.0f;
+    const int sl = threadIdx.x & 15;
+    const int nslots = gridDim.x * (blockDim.x >> 4);
+    // every 16-lane slot runs the same number of iterations, so the row-local
+    // reductions below never have a slot missing
+    const int iters = (b.nR + nslots - 1) / nslots;
+    for (int it = 0; it < iters; ++it) {
+        const int i = (blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4)) + it * nslots;
+        const bool valid = i < b.nR;
+        const int L = valid ? b.read_len[i] : 0;
         int k = bp.ntiles;
         for (int j = 0; j < bp.ntiles; ++j)
             if (L <= bp.tiles[j]) {
                 k = j;
                 break;
             }
-        if (L <= 0 || k == bp.ntiles) {
-            m.bin = L <= 0 ? -1 : 2 * bp.ntiles;
-            if (lane == 0)
-                bp.meta[i] = m;
-            continue;
-        }
-        const int off = b.read_off[i];
-        if (bp.semantics == 1) {
-            bool exotic = false;
-            for (int r = lane; r < L; r += 64)
-                exotic = exotic || row_code(bp.rows[off + r]) == kExoticCode;
-            if (__any(exotic)) {
-                m.bin = 2 * bp.ntiles + 1;
-                if (lane == 0)
-                    bp.meta[i] = m;
-                continue;
-            }
-        }
+        const bool tiled = valid && L > 0 && k < bp.ntiles;
+        const int off = tiled ? b.read_off[i] : 0;
         float eps = kEpsGpu;
-        if (bp.semantics == 1)
+        if (tiled && bp.semantics == 1)
             eps = bp.ph2pr[min((int)b.gcp[off], 127)];
-        bool ok = true;
+        bool ok = true, exotic = false;
         double nl = 1.0;
-        for (int r = lane; r < L; r += 64) {
+        for (int r = sl; tiled && r < L; r += 16) {
             const uint32_t w = bp.rows[off + r];
+            exotic = exotic || row_code(w) == kExoticCode;
             const float delta = bp.ph2pr[row_iq(w)];
             const float sigma = bp.ph2pr[row_dq(w)];
             const float alpha = 1.0f - (delta + sigma);
@@ -125,16 +138,33 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
             }
             nl *= (double)alpha;
         }
+        // reductions over the 16 lanes of this read (xor offsets < 16 stay
+        // inside the 16-lane slot)
+        int okv = ok ? 1 : 0, exv = exotic ? 1 : 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
+        for (int o = 8; o > 0; o >>= 1) {
             nl *= __shfl_xor(nl, o, 64);
-        ok = __all(ok) && bp.allow_fast != 0 && eps <= 0.5f;
-        if (ok && !(nl >= 1e-30))  // the start constant must stay a normal float
-            ok = false;
-        m.nl = ok ? (float)nl : 1.0f;
-        m.bin = 2 * k + (ok ? 0 : 1);
-        if (lane == 0)
-            bp.meta[i] = m;
+            okv &= __shfl_xor(okv, o, 64);
+            exv |= __shfl_xor(exv, o, 64);
+        }
+        if (!valid || sl != 0)
+            continue;
+        ReadMeta m;
+        m.nl = 1.0f;
+        if (L <= 0) {
+            m.bin = -1;
+        } else if (k == bp.ntiles) {
+            m.bin = 2 * bp.ntiles;
+        } else if (bp.semantics == 1 && exv) {
+            m.bin = 2 * bp.ntiles + 1;
+        } else {
+            bool fast = okv && bp.allow_fast != 0 && eps <= 0.5f;
+            if (fast && !(nl >= 1e-30))  // the start constant must stay a normal float
+                fast = false;
+            m.nl = fast ? (float)nl : 1.0f;
+            m.bin = 2 * k + (fast ? 0 : 1);
+        }
+        bp.meta[i] = m;
     }
 }
 
@@ -251,23 +281,56 @@ __device__ __forceinline__ int bin_of(const BinParams& bp, int k, int h)
     return (bp.semantics == 1 && k < 2 * bp.ntiles && bp.hapflag[h]) ? 2 * bp.ntiles + 1 : k;
 }
 
-// Per group: add its alignment counts to the (bin, hap length) histogram.
-__global__ __launch_bounds__(256) void k_count_keys(BinParams bp)
+// A group's non-empty read bins (including the empty-read slot NB), built
+// by one wave in LDS: bin id, count and start in the group's read_perm.
+struct GroupBins {
+    int n;
+    int k[kMaxBins + 1], cnt[kMaxBins + 1], st[kMaxBins + 1];
+};
+__device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, GroupBins& gb)
+{
+    const int lane = threadIdx.x;
+    const int NB1 = bp.NB + 1;
+    if (lane == 0)
+        gb.n = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < NB1; k0 += 64) {  // same trip count on every lane
+        const int k = k0 + lane;
+        const int c = k < NB1 ? bp.gb_count[(int64_t)g * NB1 + k] : 0;
+        const unsigned long long m = __ballot(c > 0);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        const int base = gb.n;
+        if (c > 0) {
+            gb.k[base + before] = k;
+            gb.cnt[base + before] = c;
+            gb.st[base + before] = bp.gb_start[(int64_t)g * NB1 + k];
+        }
+        __syncthreads();
+        if (lane == 0)
+            gb.n = base + __popcll(m);
+        __syncthreads();
+    }
+}
+
+// Per group (one wave): add its alignment counts to the (bin, hap length)
+// histogram, one lane per (hap, non-empty bin) pair.
+__global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    const int NB1 = bp.NB + 1;
+    __shared__ GroupBins gb;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        group_bins_compact(bp, g, gb);
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
-        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        const int n = gb.n;
+        for (int x = threadIdx.x; x < nh * n; x += 64) {
+            const int h = x / n, e = x - h * n;
+            const int k = gb.k[e];
             const int hl = b.hap_len[h0 + h];
-            if (hl <= 0)
+            if (hl <= 0 || k == bp.NB)
                 continue;
-            for (int k = 0; k < bp.NB; ++k) {
-                const int n = bp.gb_count[(int64_t)g * NB1 + k];
-                if (n > 0)
-                    atomicAdd(&bp.key_count[key_of(bin_of(bp, k, h0 + h), hl)], n);
-            }
+            atomicAdd(&bp.key_count[key_of(bin_of(bp, k, h0 + h), hl)], gb.cnt[e]);
         }
+        __syncthreads();
     }
 }
 
@@ -338,48 +401,60 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
     }
 }
 
-// Per group: reserve each (hap, bin) run and write its items; alignments
-// with an empty read or hap get -inf directly (align_host returns
-// log10(0) - log10(c) for them).
-__global__ __launch_bounds__(256) void k_scatter(BinParams bp)
+// Per group (one wave): reserve each (hap, non-empty bin) run -- one lane
+// per pair, haps in chunks that fit the LDS table -- then write the runs'
+// items, lanes over the reads of a run.  Alignments with an empty read or
+// hap get -inf directly (align_host returns log10(0) - log10(c) for them).
+__global__ __launch_bounds__(64) void k_scatter(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    const int NB1 = bp.NB + 1;
+    constexpr int kRuns = 2048;
+    __shared__ GroupBins gb;
+    __shared__ int run_base[kRuns];
+    const int lane = threadIdx.x;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        group_bins_compact(bp, g, gb);
         const int r0 = b.rprefix[g], nr = b.nreads[g];
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         const int64_t A = bp.align_off[g];
-        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-            const int hl = b.hap_len[h0 + h];
-            if (hl <= 0) {
-                for (int r = 0; r < nr; ++r)
-                    b.scores[A + (int64_t)r * nh + h] = -INFINITY;
-                continue;
-            }
-            for (int k = 0; k < NB1; ++k) {
-                const int n = bp.gb_count[(int64_t)g * NB1 + k];
-                if (n <= 0)
-                    continue;
-                const int st = bp.gb_start[(int64_t)g * NB1 + k];
-                if (k == bp.NB) {
-                    for (int j = 0; j < n; ++j) {
-                        const int rd = bp.read_perm[r0 + st + j];
-                        b.scores[A + (int64_t)(rd - r0) * nh + h] = -INFINITY;
-                    }
-                    continue;
+        const int n = gb.n;
+        const int hchunk = n > 0 ? max(1, kRuns / n) : nh;
+        for (int hc = 0; hc < nh; hc += hchunk) {
+            const int hcn = min(hchunk, nh - hc);
+            // reserve the runs of haps [hc, hc + hcn)
+            for (int x = lane; x < hcn * n; x += 64) {
+                const int h = hc + x / n, e = x % n;
+                const int k = gb.k[e];
+                const int hl = b.hap_len[h0 + h];
+                int base = -1;
+                if (hl > 0 && k != bp.NB) {
+                    const int kb = bin_of(bp, k, h0 + h);
+                    base = bp.bin_range[2 * kb] + atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
                 }
-                const int kb = bin_of(bp, k, h0 + h);
-                const int base = bp.bin_range[2 * kb] + atomicAdd(&bp.key_cursor[key_of(kb, hl)], n);
-                for (int j = 0; j < n; ++j) {
+                run_base[x] = base;
+            }
+            __syncthreads();
+            // write them
+            for (int x = 0; x < hcn * n; ++x) {
+                const int h = hc + x / n, e = x % n;
+                const int base = run_base[x];
+                const int cnt = gb.cnt[e], st = gb.st[e];
+                for (int j = lane; j < cnt; j += 64) {
                     const int rd = bp.read_perm[r0 + st + j];
-                    WorkItem it;
-                    it.read = rd;
-                    it.hap = h0 + h;
-                    it.out = (int32_t)(A + (int64_t)(rd - r0) * nh + h);
-                    it.pad = 0;
-                    bp.items[base + j] = it;
+                    const int64_t out = A + (int64_t)(rd - r0) * nh + h;
+                    if (base < 0) {
+                        b.scores[out] = -INFINITY;  // empty read or empty hap
+                    } else {
+                        WorkItem it;
+                        it.read = rd;
+                        it.hap = h0 + h;
+                        it.out = (int32_t)out;
+                        it.pad = 0;
+                        bp.items[base + j] = it;
+                    }
                 }
             }
+            __syncthreads();
         }
     }
 }
@@ -419,25 +494,25 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.task_ctr, (int64_t)kTaskCounters);
-    if (fixed_grids || hd.read_bytes > 0)
-        hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes, tpb, 4096)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
+        hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.hap_bytes > 0)
-        hipLaunchKernelGGL(k_encode_haps, dim3(grid(hd.hap_bytes, tpb, 2048)), dim3(tpb), 0, st, bp);
+        hipLaunchKernelGGL(k_encode_haps, dim3(grid(hd.hap_bytes >> 2, tpb, 2048)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.nH > 0)
         hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.nR > 0)  // one wave per read
-        hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.nR > 0)  // 16 lanes per read
+        hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 16, 4096)), dim3(tpb), 0, st, bp);
     if (bp.semantics == 1 && (fixed_grids || hd.nH > 0))  // one wave per hap
         hipLaunchKernelGGL(k_hap_flags, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
         hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
-        hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
+        hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
     }
     hipLaunchKernelGGL(k_scan_keys, dim3(bp.NB), dim3(1024), 0, st, bp);
     hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(64), 0, st, bp);
     if (fixed_grids || hd.G > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
+        hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
     return hipGetLastError();
 }
 

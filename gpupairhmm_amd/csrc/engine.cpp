@@ -185,6 +185,15 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
                              !v.gcp_quals)) ||
         (v.num_haps > 0 && !v.hap_data))
         return set_error(PHMM_EINVAL, "score_device: null device pointer in batch view");
+    // the encoders read sequence and quality bytes as dwords: every sequence
+    // is padded to a multiple of 4 bytes (4 - len % 4, as the parser does)
+    auto misaligned = [](const void* q) { return ((uintptr_t)q & 3) != 0; };
+    if ((v.read_bytes & 3) || (v.hap_bytes & 3) || misaligned(v.read_data) ||
+        misaligned(v.base_quals) || misaligned(v.ins_quals) || misaligned(v.del_quals) ||
+        misaligned(v.hap_data))
+        return set_error(PHMM_EINVAL,
+                         "score_device: sequence arrays must be 4-byte aligned with padded lengths "
+                         "(byte counts multiples of 4)");
     const int ntiles = (int)tiles.size();
     const int NB = 2 * ntiles + 2;  // + multi-tile bin + exact bin
     const int G = v.num_groups;

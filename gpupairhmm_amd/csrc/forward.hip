@@ -478,6 +478,8 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
     __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
     SetLane sl;
     const int total = set_lanes(bs, 0, sl);
+    if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
+        return;
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
         const int k = set_find(sl, task);
         const int bin = __shfl(sl.bin, k, 64);
