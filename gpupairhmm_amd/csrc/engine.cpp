@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -668,38 +670,92 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
     int32_t* ho = reinterpret_cast<int32_t*>(P + L.ho);
     int32_t* rp = reinterpret_cast<int32_t*>(P + L.rp);
     int32_t* hp = reinterpret_cast<int32_t*>(P + L.hp);
-    int64_t rbase = 0, hbase = 0;
-    int32_t rcount = 0, hcount = 0, gcount = 0;
+    // where each part lands in the chunk
+    struct Base {
+        int64_t rb, hb;
+        int32_t r, h, g;
+    };
+    std::vector<Base> base((size_t)nparts);
+    {
+        Base cur{0, 0, 0, 0, 0};
+        for (int k = 0; k < nparts; ++k) {
+            base[(size_t)k] = cur;
+            const Span& x = sp[(size_t)k];
+            cur.rb += x.rb1 - x.rb0;
+            cur.hb += x.hb1 - x.hb0;
+            cur.r += x.r1 - x.r0;
+            cur.h += x.h1 - x.h0;
+            cur.g += parts[k].g1 - parts[k].g0;
+        }
+    }
     rp[0] = 0;
     hp[0] = 0;
+    // the bulk bytes as slices of at most 2 MiB, spread over host threads
+    // for large chunks; the per-sequence bookkeeping per part
+    struct Copy {
+        char* dst;
+        const void* src;
+        size_t n;
+    };
+    std::vector<Copy> copies;
     for (int k = 0; k < nparts; ++k) {
+        const phmm_batch_view& v = *parts[k].v;
+        const Span& x = sp[(size_t)k];
+        const Base& bs = base[(size_t)k];
+        const size_t rb = (size_t)(x.rb1 - x.rb0), hb = (size_t)(x.hb1 - x.hb0);
+        const Copy whole[6] = {{P + L.read_data + bs.rb, v.read_data + x.rb0, rb},
+                               {P + L.bq + bs.rb, v.base_quals + x.rb0, rb},
+                               {P + L.iq + bs.rb, v.ins_quals + x.rb0, rb},
+                               {P + L.dq + bs.rb, v.del_quals + x.rb0, rb},
+                               {P + L.gq + bs.rb, v.gcp_quals + x.rb0, rb},
+                               {P + L.hap_data + bs.hb, v.hap_data + x.hb0, hb}};
+        constexpr size_t kSlice = (size_t)2 << 20;
+        for (const Copy& c : whole)
+            for (size_t o = 0; o < c.n; o += kSlice)
+                copies.push_back({c.dst + o, static_cast<const char*>(c.src) + o,
+                                  std::min(kSlice, c.n - o)});
+    }
+    auto bookkeeping = [&](int k) {
         const phmm_batch_view& v = *parts[k].v;
         const int g0 = parts[k].g0, g1 = parts[k].g1;
         const Span& x = sp[(size_t)k];
-        const size_t rb = (size_t)(x.rb1 - x.rb0), hb = (size_t)(x.hb1 - x.hb0);
-        std::memcpy(P + L.read_data + rbase, v.read_data + x.rb0, rb);
-        std::memcpy(P + L.bq + rbase, v.base_quals + x.rb0, rb);
-        std::memcpy(P + L.iq + rbase, v.ins_quals + x.rb0, rb);
-        std::memcpy(P + L.dq + rbase, v.del_quals + x.rb0, rb);
-        std::memcpy(P + L.gq + rbase, v.gcp_quals + x.rb0, rb);
-        std::memcpy(P + L.hap_data + hbase, v.hap_data + x.hb0, hb);
-        std::memcpy(P + L.rl + 4 * (size_t)rcount, v.read_lengths + x.r0, 4 * (size_t)(x.r1 - x.r0));
-        std::memcpy(P + L.hl + 4 * (size_t)hcount, v.hap_lengths + x.h0, 4 * (size_t)(x.h1 - x.h0));
+        const Base& bs = base[(size_t)k];
+        std::memcpy(P + L.rl + 4 * (size_t)bs.r, v.read_lengths + x.r0, 4 * (size_t)(x.r1 - x.r0));
+        std::memcpy(P + L.hl + 4 * (size_t)bs.h, v.hap_lengths + x.h0, 4 * (size_t)(x.h1 - x.h0));
         for (int i = x.r0; i < x.r1; ++i)
-            ro[rcount + (i - x.r0)] = (int32_t)(rbase + v.read_offsets[i] - x.rb0);
+            ro[bs.r + (i - x.r0)] = (int32_t)(bs.rb + v.read_offsets[i] - x.rb0);
         for (int i = x.h0; i < x.h1; ++i)
-            ho[hcount + (i - x.h0)] = (int32_t)(hbase + v.hap_offsets[i] - x.hb0);
-        std::memcpy(P + L.nr + 4 * (size_t)gcount, v.num_reads_per_group + g0, 4 * (size_t)(g1 - g0));
-        std::memcpy(P + L.nh + 4 * (size_t)gcount, v.num_haps_per_group + g0, 4 * (size_t)(g1 - g0));
+            ho[bs.h + (i - x.h0)] = (int32_t)(bs.hb + v.hap_offsets[i] - x.hb0);
+        std::memcpy(P + L.nr + 4 * (size_t)bs.g, v.num_reads_per_group + g0, 4 * (size_t)(g1 - g0));
+        std::memcpy(P + L.nh + 4 * (size_t)bs.g, v.num_haps_per_group + g0, 4 * (size_t)(g1 - g0));
         for (int g = g0; g < g1; ++g) {
-            rp[gcount + (g - g0) + 1] = rcount + (v.reads_prefix[g + 1] - x.r0);
-            hp[gcount + (g - g0) + 1] = hcount + (v.haps_prefix[g + 1] - x.h0);
+            rp[bs.g + (g - g0) + 1] = bs.r + (v.reads_prefix[g + 1] - x.r0);
+            hp[bs.g + (g - g0) + 1] = bs.h + (v.haps_prefix[g + 1] - x.h0);
         }
-        rbase += (int64_t)rb;
-        hbase += (int64_t)hb;
-        rcount += x.r1 - x.r0;
-        hcount += x.h1 - x.h0;
-        gcount += g1 - g0;
+    };
+    const size_t bulk = 5 * (size_t)rbytes + (size_t)hbytes;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = bulk >= ((size_t)16 << 20) ? (int)std::min(8u, hw > 1 ? hw : 1u) : 1;
+    if (nthreads <= 1) {
+        for (const Copy& c : copies)
+            std::memcpy(c.dst, c.src, c.n);
+        for (int k = 0; k < nparts; ++k)
+            bookkeeping(k);
+    } else {
+        std::atomic<size_t> next_copy{0};
+        std::atomic<int> next_part{0};
+        auto work = [&]() {
+            for (size_t i = next_copy++; i < copies.size(); i = next_copy++)
+                std::memcpy(copies[i].dst, copies[i].src, copies[i].n);
+            for (int k = next_part++; k < nparts; k = next_part++)
+                bookkeeping(k);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthreads; ++t)
+            pool.emplace_back(work);
+        work();
+        for (auto& th : pool)
+            th.join();
     }
     ro[nR] = (int32_t)rbytes;
     ho[nH] = (int32_t)hbytes;
