@@ -273,13 +273,6 @@ __global__ __launch_bounds__(256) void k_hap_flags(BinParams bp)
     }
 }
 
-// The bin an alignment of read bin k and hap h goes to: a hap with bytes
-// other than A/C/G/T/N (CPU semantics) sends its tile-bin alignments to the
-// exact bin.
-__device__ __forceinline__ int bin_of(const BinParams& bp, int k, int h)
-{
-    return (bp.semantics == 1 && k < 2 * bp.ntiles && bp.hapflag[h]) ? 2 * bp.ntiles + 1 : k;
-}
 
 // A group's non-empty read bins (including the empty-read slot NB), built
 // by one wave in LDS: bin id, count and start in the group's read_perm.
@@ -312,23 +305,124 @@ __device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, G
     }
 }
 
-// Per group (one wave): add its alignment counts to the (bin, hap length)
-// histogram, one lane per (hap, non-empty bin) pair.
+// Per-group hap summary, uniform across the wave: haps with a sequence and
+// those of them that go to the exact bin (CPU semantics); the others are
+// cut, in order, into segments of at most `budget` columns (at least one hap
+// each) -- the super-tasks' hap ranges and their work (sort key).
+constexpr int kMaxSegs = 512;
+struct GroupHaps {
+    int nonempty, exotic, nseg;
+    int hs[kMaxSegs], he[kMaxSegs], work[kMaxSegs];
+};
+__device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc& b, int h0, int nh,
+                                           GroupHaps& gh)
+{
+    int ne = 0, ex = 0, tot = 0;
+    for (int h = threadIdx.x; h < nh; h += 64) {
+        const int hl = b.hap_len[h0 + h];
+        if (hl > 0) {
+            ++ne;
+            if (bp.semantics == 1 && bp.hapflag[h0 + h])
+                ++ex;
+            else
+                tot += hl;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ne += __shfl_xor(ne, o, 64);
+        ex += __shfl_xor(ex, o, 64);
+        tot += __shfl_xor(tot, o, 64);
+    }
+    // greedy segments number at most 2*tot/budget + 1: a larger budget where
+    // they would not fit the table
+    const int budget = max(b.col_budget, (2 * tot + kMaxSegs - 2) / (kMaxSegs - 1));
+    if (threadIdx.x == 0) {
+        gh.nonempty = ne;
+        gh.exotic = ex;
+        int n = 0, cur = 0, start = 0;
+        for (int h = 0; h < nh; ++h) {
+            const int hl = b.hap_len[h0 + h];
+            if (hl <= 0 || (bp.semantics == 1 && bp.hapflag[h0 + h]))
+                continue;
+            if (cur > 0 && cur + hl > budget) {
+                gh.hs[n] = start;
+                gh.he[n] = h;
+                gh.work[n] = cur;
+                ++n;
+                cur = 0;
+                start = h;
+            }
+            if (cur == 0)
+                start = h;
+            cur += hl;
+        }
+        if (cur > 0) {
+            gh.hs[n] = start;
+            gh.he[n] = nh;
+            gh.work[n] = cur;
+            ++n;
+        }
+        gh.nseg = n;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool is_tile_bin(const BinParams& bp, int k) { return k < 2 * bp.ntiles; }
+__device__ __forceinline__ int tasks_of(const BinParams& bp, int k, int reads)
+{
+    const int W = 64 / bp.lanes[k >> 1];
+    return (reads + W - 1) / W;
+}
+
+// Per group (one wave): the sort-key histogram and the per-bin alignment
+// counts.  A tile bin gets one key (bin, total hap length) per group, counted
+// in super-tasks; the multi-tile and exact bins get one key (bin, hap length)
+// per (hap, bin) run, counted in alignments.
 __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     __shared__ GroupBins gb;
+    __shared__ GroupHaps gh;
+    const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
         group_bins_compact(bp, g, gb);
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
+        group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
+        // per bin entry: alignment counts
+        for (int e = threadIdx.x; e < n; e += 64) {
+            const int k = gb.k[e], c = gb.cnt[e];
+            if (k == bp.NB)
+                continue;
+            if (is_tile_bin(bp, k)) {
+                const int tiled = gh.nonempty - gh.exotic;
+                if (tiled > 0)
+                    atomicAdd(&bp.align_range[2 * k + 1], c * tiled);
+                if (gh.exotic > 0)
+                    atomicAdd(&bp.align_range[2 * eb + 1], c * gh.exotic);
+            } else if (gh.nonempty > 0) {
+                atomicAdd(&bp.align_range[2 * k + 1], c * gh.nonempty);
+            }
+        }
+        // per (tile bin entry, hap segment): its super-tasks' key
+        for (int x = threadIdx.x; x < n * gh.nseg; x += 64) {
+            const int e = x / gh.nseg, sg = x - e * gh.nseg;
+            const int k = gb.k[e];
+            if (k != bp.NB && is_tile_bin(bp, k))
+                atomicAdd(&bp.key_count[key_of(k, gh.work[sg])], tasks_of(bp, k, gb.cnt[e]));
+        }
+        // per (hap, bin) item run of the one-wave-per-alignment bins
         for (int x = threadIdx.x; x < nh * n; x += 64) {
             const int h = x / n, e = x - h * n;
             const int k = gb.k[e];
             const int hl = b.hap_len[h0 + h];
             if (hl <= 0 || k == bp.NB)
                 continue;
-            atomicAdd(&bp.key_count[key_of(bin_of(bp, k, h0 + h), hl)], gb.cnt[e]);
+            const bool exotic = bp.semantics == 1 && bp.hapflag[h0 + h];
+            if (is_tile_bin(bp, k) && !exotic)
+                continue;  // a super-task covers it
+            atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl)], gb.cnt[e]);
         }
         __syncthreads();
     }
@@ -382,62 +476,108 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
     }
 }
 
-// Bin begins: exclusive scan of the bin counts (NB <= 65, one wave).
-__global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
+// Bin begins: exclusive scans of the bin counts -- in work units
+// (bin_range) and in alignments (align_range) -- one wave, NB <= 66.
+__device__ __forceinline__ void scan_pairs(int32_t* range, int NB)
 {
     int carry = 0;
-    for (int base = 0; base < bp.NB; base += 64) {
+    for (int base = 0; base < NB; base += 64) {
         const int k = base + (int)threadIdx.x;
-        const int v = k < bp.NB ? bp.bin_range[2 * k + 1] : 0;
+        const int v = k < NB ? range[2 * k + 1] : 0;
         int x = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o, 64);
             if ((int)threadIdx.x >= o)
                 x += y;
         }
-        if (k < bp.NB)
-            bp.bin_range[2 * k] = carry + x - v;
+        if (k < NB)
+            range[2 * k] = carry + x - v;
         carry += __shfl(x, 63, 64);
     }
 }
 
-// Per group (one wave): reserve each (hap, non-empty bin) run -- one lane
-// per pair, haps in chunks that fit the LDS table -- then write the runs'
-// items, lanes over the reads of a run.  Alignments with an empty read or
-// hap get -inf directly (align_host returns log10(0) - log10(c) for them).
+__global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
+{
+    scan_pairs(bp.bin_range, bp.NB);
+    scan_pairs(bp.align_range, bp.NB);
+}
+
+// Per group (one wave): write the work lists.  Tile bins: the group's reads
+// of the bin in chunks of 64/lanes, one super-task each (all haps).  The
+// multi-tile and exact bins: one item per alignment, reserved per (hap, bin)
+// run -- one lane per pair, haps in chunks that fit the LDS table -- and
+// written with lanes over the reads of a run.  Alignments with an empty read
+// or hap get -inf directly (align_host returns log10(0) - log10(c) for them).
 __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     constexpr int kRuns = 2048;
+    constexpr int kSkip = -2;  // run covered by a super-task
     __shared__ GroupBins gb;
+    __shared__ GroupHaps gh;
     __shared__ int run_base[kRuns];
     const int lane = threadIdx.x;
+    const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
         group_bins_compact(bp, g, gb);
-        const int r0 = b.rprefix[g], nr = b.nreads[g];
+        const int r0 = b.rprefix[g];
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         const int64_t A = bp.align_off[g];
+        group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
+        // super-tasks of the tile bins: read chunks x hap segments
+        for (int e = 0; e < n; ++e) {
+            const int k = gb.k[e];
+            if (k == bp.NB || !is_tile_bin(bp, k))
+                continue;
+            const int c = gb.cnt[e], W = 64 / bp.lanes[k >> 1];
+            const int nt = (c + W - 1) / W;
+            for (int sg = 0; sg < gh.nseg; ++sg) {
+                int base = 0;
+                if (lane == 0)
+                    base = bp.bin_range[2 * k] + atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], nt);
+                base = __shfl(base, 0, 64);
+                for (int j = lane; j < nt; j += 64) {
+                    SuperTask t;
+                    t.perm = r0 + gb.st[e] + j * W;
+                    t.nc = min(W, c - j * W);
+                    t.h0 = h0;
+                    t.nh = nh;
+                    t.hs = gh.hs[sg];
+                    t.he = gh.he[sg];
+                    t.r0 = r0;
+                    t.aoff = (int32_t)A;
+                    bp.st[base + j] = t;
+                }
+            }
+        }
+        // item runs and -inf
         const int hchunk = n > 0 ? max(1, kRuns / n) : nh;
         for (int hc = 0; hc < nh; hc += hchunk) {
             const int hcn = min(hchunk, nh - hc);
-            // reserve the runs of haps [hc, hc + hcn)
             for (int x = lane; x < hcn * n; x += 64) {
                 const int h = hc + x / n, e = x % n;
                 const int k = gb.k[e];
                 const int hl = b.hap_len[h0 + h];
-                int base = -1;
+                int base = -1;  // -inf
                 if (hl > 0 && k != bp.NB) {
-                    const int kb = bin_of(bp, k, h0 + h);
-                    base = bp.bin_range[2 * kb] + atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
+                    const bool exotic = bp.semantics == 1 && bp.hapflag[h0 + h];
+                    if (is_tile_bin(bp, k) && !exotic) {
+                        base = kSkip;
+                    } else {
+                        const int kb = is_tile_bin(bp, k) ? eb : k;
+                        base = bp.bin_range[2 * kb] +
+                               atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
+                    }
                 }
                 run_base[x] = base;
             }
             __syncthreads();
-            // write them
             for (int x = 0; x < hcn * n; ++x) {
-                const int h = hc + x / n, e = x % n;
                 const int base = run_base[x];
+                if (base == kSkip)
+                    continue;
+                const int h = hc + x / n, e = x % n;
                 const int cnt = gb.cnt[e], st = gb.st[e];
                 for (int j = lane; j < cnt; j += 64) {
                     const int rd = bp.read_perm[r0 + st + j];
@@ -494,6 +634,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.task_ctr, (int64_t)kTaskCounters);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(256), 0, st, bp.align_range, (int64_t)2 * bp.NB);
     if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
         hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.hap_bytes > 0)

@@ -63,7 +63,8 @@ Engine::~Engine()
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
     for (DevBuf* b : {&rows, &hapc, &meta, &gb_count, &gb_start, &read_perm, &align_off, &key_count,
-                      &key_cursor, &bin_range, &smallints, &items, &rescue_list, &final_list, &spill,
+                      &key_cursor, &bin_range, &align_range, &smallints, &items, &supertasks,
+                      &rescue_items, &final_items, &spill,
                       &hapflag})
         if (b->p)
             hipFree(b->p);
@@ -213,8 +214,12 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NB, "bin ranges")) ||
         (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
-        (rc = ensure(rescue_list, sizeof(int32_t) * (size_t)nA, "rescue list")) ||
-        (rc = ensure(final_list, sizeof(int32_t) * (size_t)nA, "final list")) ||
+        (rc = ensure(align_range, sizeof(int32_t) * 2 * (size_t)NB, "alignment ranges")) ||
+        // each super-task covers at least one (read, hap) alignment and no
+        // two cover the same one: at most num_alignments of them
+        (rc = ensure(supertasks, sizeof(SuperTask) * (size_t)std::max<int64_t>(nA, 1), "super-tasks")) ||
+        (rc = ensure(rescue_items, sizeof(WorkItem) * (size_t)nA, "rescue list")) ||
+        (rc = ensure(final_items, sizeof(WorkItem) * (size_t)nA, "final list")) ||
         (rc = ensure(desc, sizeof(BatchDesc), "batch descriptor")) ||
         (rc = ensure(hapflag, (size_t)std::max(v.num_haps, 1), "hap flags")))
         return rc;
@@ -255,6 +260,13 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     hd.G = G;
     hd.nR = v.num_reads;
     hd.nH = v.num_haps;
+    // super-task segments: about 32k tasks for the batch (16 per resident
+    // wave), so long or few haps fall back to one hap per task
+    {
+        const double mean_hl = v.num_haps > 0 ? (double)v.hap_bytes / v.num_haps : 0.0;
+        const double cols = (double)nA * mean_hl;
+        hd.col_budget = (int32_t)std::min(8192.0, std::max(256.0, cols / (8.0 * 32768.0)));
+    }
     hd.read_bytes = v.read_bytes;
     hd.hap_bytes = v.hap_bytes;
     hd.num_alignments = nA;
@@ -296,23 +308,28 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.task_ctr = sm + 128;
     bp.hapflag = static_cast<uint8_t*>(hapflag.p);
     bp.items = static_cast<WorkItem*>(items.p);
+    bp.st = static_cast<SuperTask*>(supertasks.p);
+    bp.align_range = static_cast<int32_t*>(align_range.p);
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
-    for (int i = 0; i < ntiles; ++i)
+    for (int i = 0; i < ntiles; ++i) {
         bp.tiles[i] = tiles[(size_t)i].max_read_len;
+        bp.lanes[i] = tiles[(size_t)i].lanes;
+    }
     bp.semantics = opt.semantics;
     bp.allow_fast = 1;
 
     FwdParams fp{};
     fp.bd = bp.bd;
-    fp.items = bp.items;
     fp.rows = bp.rows;
     fp.hapc = bp.hapc;
     fp.meta = bp.meta;
+    fp.read_perm = bp.read_perm;
+    fp.hapflag = bp.hapflag;
     fp.ph2pr = d_ph2pr;
-    fp.rescue_list = static_cast<int32_t*>(rescue_list.p);
+    fp.rescue_items = static_cast<WorkItem*>(rescue_items.p);
     fp.rescue_counts = bp.rescue_counts;
-    fp.final_list = static_cast<int32_t*>(final_list.p);
+    fp.final_items = static_cast<WorkItem*>(final_items.p);
     fp.final_count = bp.final_count;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
@@ -340,6 +357,9 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
             // gpuworker.cuh:852-854)
             if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NB,
                                            hipMemcpyDeviceToHost, st),
+                            "bin range readback")) ||
+                (rc = check(hipMemcpyAsync(h_bin_range + 2 * kMaxBins, bp.align_range,
+                                           sizeof(int32_t) * 2 * NB, hipMemcpyDeviceToHost, st),
                             "bin range readback")) ||
                 (rc = check(hipStreamSynchronize(st), "bin range sync")))
                 return rc;
@@ -395,32 +415,49 @@ int Engine::enqueue_prep(const BinParams& bp, const BatchDesc& hd, bool fixed, h
     return check(run_binning(bp, hd, fixed, st), "binning kernels");
 }
 
-// Pass 1: FP32 tile kernels (results below the rescue threshold go to the
-// bin's rescue list) and the FP32 multi-tile kernel for long reads.  Plain:
-// one exact launch per non-empty bin.  Batched: one persistent launch per
-// kernel template covering all of its bins, larger tiles first.
+// Pass 1: FP32 tile kernels over the tile bins' super-tasks (results below
+// the rescue threshold go to the bin's rescue list) and the one-wave kernel
+// over the multi-tile and exact bins' items.  Batched: one persistent launch
+// per kernel template covering all of its bins, larger tiles first.  Plain:
+// one launch per non-empty bin after the host has read the bin sizes back
+// (the reference's per-partition launches).
 int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
 {
     const int ntiles = (int)tiles.size();
     const int mb = 2 * ntiles;  // multi-tile bin
+    const int eb = mb + 1;      // exact bin
     float* spill32 = static_cast<float*>(spill.p);
     const int64_t n32 = (int64_t)(spill.cap / sizeof(float));
+    const int32_t* h_align = h_bin_range + 2 * kMaxBins;
     int rc;
     int ctr = 0;
+    auto tile_set = [&](BinSet& bs) {
+        bs.st = bp.st;
+        bs.range = bp.bin_range;
+        bs.counts = bp.bin_range + 1;
+        bs.count_stride = 2;
+        bs.seg = bp.align_range;
+    };
     if (plain) {
-        for (int b = 0; b < 2 * ntiles; ++b) {
-            const int count = h_bin_range[2 * b + 1];
-            if (count <= 0)
+        for (int b = 2 * ntiles - 1; b >= 0; --b) {
+            const int tasks = h_bin_range[2 * b + 1];
+            if (tasks <= 0)
                 continue;
             const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
-            TileLaunch tl;
-            tl.G = tc.lanes;
-            tl.bin = b;
-            tl.begin = h_bin_range[2 * b];
-            tl.count = count;
             if (b % 2 == 0)
-                stats.fast_alignments += count;
-            if ((rc = check(launch_fwd_tile(tc.rows, (b % 2) == 0, fp, tl, st), "forward kernel")))
+                stats.fast_alignments += h_align[2 * b + 1];
+            BinSet bs;
+            bs.n = 1;
+            bs.bin[0] = b;
+            bs.lanes[0] = tc.lanes;
+            tile_set(bs);
+            if (ctr >= kTaskCounters - 2)
+                return set_error(PHMM_ERANGE, "too many set launches");
+            bs.next_task = bp.task_ctr + ctr++;
+            const int grid =
+                std::min(tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false));
+            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fp, bs, grid, st),
+                            "forward kernel")))
                 return rc;
         }
     } else {
@@ -440,10 +477,8 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                     bs.lanes[bs.n] = tiles[(size_t)(b / 2)].lanes;
                     ++bs.n;
                 }
-            bs.range = bp.bin_range;
-            bs.counts = bp.bin_range + 1;
-            bs.count_stride = 2;
-            if (ctr >= kTaskCounters)
+            tile_set(bs);
+            if (ctr >= kTaskCounters - 2)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st), "forward kernel")))
@@ -451,7 +486,6 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
         }
     }
     // the multi-tile bin and the exact bin: one wave per alignment
-    const int eb = mb + 1;
     if (plain) {
         const int n = h_bin_range[2 * mb + 1] + h_bin_range[2 * eb + 1];
         if (n <= 0)
@@ -463,6 +497,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
     ws.bin[0] = mb;
     ws.bin[1] = eb;
     ws.lanes[0] = ws.lanes[1] = 64;
+    ws.items = bp.items;
     ws.range = bp.bin_range;
     ws.counts = bp.bin_range + 1;
     ws.count_stride = 2;
@@ -472,25 +507,26 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                  "multi-tile kernel");
 }
 
-// Pass 2: FP64 recompute of each bin's rescue list with an FP64 tile of at
-// least the bin's size (tiles above 512 rows: one wave per alignment), one
-// launch per FP64 template; pass 3: FP64 waves for anything still not
+// Pass 2: FP64 recompute of each tile bin's rescue list with an FP64 tile
+// of at least the bin's size (tiles above 512 rows: one wave per alignment),
+// one launch per FP64 template; pass 3: FP64 waves for anything still not
 // finished.
 int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
 {
     const int ntiles = (int)tiles.size();
     double* spill64 = static_cast<double*>(spill.p);
     const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
+    const int32_t* h_align = h_bin_range + 2 * kMaxBins;
     int rc;
     BinSet set4, set8, wave;
     for (BinSet* x : {&set4, &set8, &wave}) {
-        x->list = fp.rescue_list;
-        x->range = bp.bin_range;
+        x->items = fp.rescue_items;
+        x->range = bp.align_range;
         x->counts = fp.rescue_counts;
         x->count_stride = 1;
     }
     for (int b = 2 * ntiles - 1; b >= 0; --b) {
-        if (plain && h_bin_range[2 * b + 1] <= 0)
+        if (plain && h_align[2 * b + 1] <= 0)
             continue;
         const int tile = tiles[(size_t)(b / 2)].max_read_len + 1;  // rows the FP64 tile must hold
         BinSet& x = tile > 512 ? wave : tile <= 64 ? set4 : set8;
@@ -511,7 +547,7 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     fin.n = 1;
     fin.bin[0] = 0;
     fin.lanes[0] = 64;
-    fin.list = fp.final_list;
+    fin.items = fp.final_items;
     fin.counts = fp.final_count;
     fin.count_stride = 0;
     if ((rc = check(launch_fwd_wave_f64(fp, fin, bp.hmax, spill64, n64, bp.error_flag, num_cus * 8,
@@ -963,7 +999,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
         (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipHostMalloc(&e.h_bin_range, 2 * sizeof(int32_t) * phmm::kMaxBins,
+        (rc = e.check(hipHostMalloc(&e.h_bin_range, 4 * sizeof(int32_t) * phmm::kMaxBins,
                                     hipHostMallocDefault),
                       "pinned")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")))

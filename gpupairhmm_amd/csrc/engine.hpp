@@ -38,8 +38,9 @@ struct Engine {
 
     // scratch (grow-only)
     DevBuf rows, hapc, meta, gb_count, gb_start, read_perm, align_off, key_count, key_cursor,
-        bin_range, smallints, items, rescue_list, final_list, spill, hapflag;
-    int32_t* h_bin_range = nullptr;   // pinned, 2*NB
+        bin_range, align_range, smallints, items, supertasks, rescue_items, final_items, spill,
+        hapflag;
+    int32_t* h_bin_range = nullptr;   // pinned, 2*NB work ranges then 2*NB alignment ranges
     int32_t* h_small = nullptr;       // pinned, small readbacks
     // event pairs around the forward kernels of timed calls
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_pending;

@@ -8,7 +8,7 @@
 //   row 0: M = I = 0, D = (FLT_MAX/16)/H;  column 0 (r >= 1): 0
 //   score = log10(sum_c M[R][c] + I[R][c]) - log10(FLT_MAX/16)
 //
-// k_fwd_tile -- the hot path.  A wave holds 64/G alignments ("groups") of G
+// k_fwd_tile_set -- the hot path.  A wave holds 64/G alignments ("groups") of G
 // lanes; lane t of a group holds read rows [t*NR, t*NR+NR) in registers and
 // the haplotype streams through the group one column per step (anti-diagonal
 // wavefront).  Row-above values cross lanes with DPP wave_shr:1, a group's
@@ -124,45 +124,35 @@ struct TileTraits<double> {
     __device__ static double shr(double x) { return shr1d(x); }
 };
 
-// One task: W = 64/G alignments of a bin.  Item k of the bin is
-// items[list ? list[begin + k] : begin + k].
-//
-// T = float is the first pass: results whose FP32 sum (in the reference's
-// units) is below rescue_min are appended to the bin's rescue list
-// (rescue_list[begin + n], n counted in rescue_counts[bin]).  T = double is
-// the FP64 recompute of those lists (start constant 2^1000/H); a result that
-// is still not a positive finite number goes to the final list.
+// Per-lane state of a tile task that depends only on the read: its rows'
+// coefficients (the emission values of the rows live in the LDS table).
+template <typename T, int NR>
+struct TileRead {
+    T c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e -; SAFE: alpha w delta' e
+    T eps, beta;
+    int L;
+    float nl;
+};
+
+// This lane's rows of `read` (inactive lanes: an empty read): coefficients in
+// R, emission values in the LDS table.
 template <typename T, int NR, bool FAST>
-__device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileTraits<T>::V* tab,
-                                              const int32_t* list, int begin, int count, int G,
-                                              int task, int bin)
+__device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTraits<T>::V* tab,
+                                           bool active, int read, int t, TileRead<T, NR>& R)
 {
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
-    using V = typename TT::V;
     constexpr int K = TT::K;
     constexpr int NQ = NR / K;
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
-    const int W = 64 / G;
-    const int grp = lane / G;
-    const int t = lane - grp * G;
-    const bool first = (t == 0);
-    const int local = task * W + grp;
-    const bool active = grp < W && local < count;
-
-    int L = 0, HL = 0, roff = 0, hoff = 0, out = -1, idx = -1;
+    int L = 0, roff = 0;
     float nl = 1.0f;
     if (active) {
-        idx = list ? list[begin + local] : begin + local;
-        const WorkItem it = p.items[idx];
-        out = it.out;
-        L = bd.read_len[it.read];
-        HL = bd.hap_len[it.hap];
-        roff = bd.read_off[it.read];
-        hoff = bd.hap_off[it.hap];
+        L = bd.read_len[read];
+        roff = bd.read_off[read];
         if (FAST)
-            nl = p.meta[it.read].nl;
+            nl = p.meta[read].nl;
     }
     T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
     if (p.semantics == 1 && active && L > 0) {
@@ -170,10 +160,13 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
         eps = (T)e;
         beta = F32 ? (T)(float)(1.0 - (double)e) : (T)1 - (T)e;
     }
+    R.eps = eps;
+    R.beta = beta;
+    R.L = L;
+    R.nl = nl;
 
     // ---- per-row coefficients and the emission table -------------------
     const int r0 = t * NR;
-    T c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e -; SAFE: alpha w delta' e
     {
         T pdelta = 0, psigma = 0, palpha = 1;
         if (r0 >= 1 && r0 - 1 < L) {
@@ -198,20 +191,20 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 lx[j] = qdiv(pb, (T)3);
                 if (FAST) {
                     if (ri == 0) {
-                        c0[j] = 0;
-                        c1[j] = qdiv(beta, alpha);
-                        c2[j] = 0;
+                        R.c0[j] = 0;
+                        R.c1[j] = qdiv(beta, alpha);
+                        R.c2[j] = 0;
                     } else {
-                        c0[j] = qdiv(beta * pdelta, palpha * alpha);
-                        c1[j] = qdiv(beta * psigma, alpha);
-                        c2[j] = qdiv(eps * pdelta, delta * palpha);
+                        R.c0[j] = qdiv(beta * pdelta, palpha * alpha);
+                        R.c1[j] = qdiv(beta * psigma, alpha);
+                        R.c2[j] = qdiv(eps * pdelta, delta * palpha);
                     }
-                    c3[j] = 0;
+                    R.c3[j] = 0;
                 } else {
-                    c0[j] = alpha;
-                    c1[j] = ri == 0 ? beta : beta * psigma;
-                    c2[j] = beta * delta;
-                    c3[j] = eps;
+                    R.c0[j] = alpha;
+                    R.c1[j] = ri == 0 ? beta : beta * psigma;
+                    R.c2[j] = beta * delta;
+                    R.c3[j] = eps;
                 }
                 pdelta = delta;
                 psigma = sigma;
@@ -222,15 +215,15 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 lx[j] = 0;
                 const bool firstoob = (ri == L);
                 if (FAST) {
-                    c0[j] = 0;
-                    c1[j] = 0;
-                    c2[j] = firstoob ? qdiv(pdelta, palpha) : (T)1;
-                    c3[j] = 0;
+                    R.c0[j] = 0;
+                    R.c1[j] = 0;
+                    R.c2[j] = firstoob ? qdiv(pdelta, palpha) : (T)1;
+                    R.c3[j] = 0;
                 } else {
-                    c0[j] = 0;
-                    c1[j] = 0;
-                    c2[j] = firstoob ? beta : (T)0;
-                    c3[j] = 1;
+                    R.c0[j] = 0;
+                    R.c1[j] = 0;
+                    R.c2[j] = firstoob ? beta : (T)0;
+                    R.c3[j] = 1;
                 }
             }
         }
@@ -250,6 +243,33 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
             }
         }
     }
+
+}
+
+// One sweep of this group's read (R) against a hap of length HL at hoff;
+// the group's result lane writes the score of `it`.  T = float is the first
+// pass: results whose FP32 sum (in the reference's units) is below
+// rescue_min are appended to the bin's rescue list (rescue_items[rescue_base
+// + n], n counted in rescue_counts[bin]).  T = double is the FP64 recompute
+// of those lists (start constant 2^1000/H); a result whose sum is not far
+// above double's floor goes to the final list.
+template <typename T, int NR, bool FAST>
+__device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
+                                           const TileRead<T, NR>& R, bool active, int G, int t,
+                                           int HL, int hoff, const WorkItem& it, int rescue_base,
+                                           int bin)
+{
+    const BatchDesc& bd = *p.bd;
+    using TT = TileTraits<T>;
+    using V = typename TT::V;
+    constexpr int K = TT::K;
+    constexpr int NQ = NR / K;
+    constexpr bool F32 = sizeof(T) == 4;
+    const int lane = threadIdx.x;
+    const bool first = (t == 0);
+    const T eps = R.eps, beta = R.beta;
+    const int L = R.L;
+    const float nl = R.nl;
 
     // ---- wavefront sweep -------------------------------------------------
     T M[NR], D[NR];
@@ -334,13 +354,13 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                     const T mo = M[j], dov = D[j];
                     T t1;
                     if (FAST) {
-                        t1 = fma(c1[j], dp, mp);
-                        t1 = fma(c0[j], chain, t1);
-                        chain = fma(c2[j], chain, mp);
+                        t1 = fma(R.c1[j], dp, mp);
+                        t1 = fma(R.c0[j], chain, t1);
+                        chain = fma(R.c2[j], chain, mp);
                     } else {
-                        t1 = fma(c1[j], dp, chain);
-                        t1 = fma(c0[j], mp, t1);
-                        chain = fma(c2[j], mp, c3[j] * chain);
+                        t1 = fma(R.c1[j], dp, chain);
+                        t1 = fma(R.c0[j], mp, t1);
+                        chain = fma(R.c2[j], mp, R.c3[j] * chain);
                     }
                     M[j] = lv[k] * t1;
                     D[j] = fma(eps, dov, mo);
@@ -378,44 +398,90 @@ __device__ __forceinline__ void fwd_tile_task(const FwdParams& p, typename TileT
                 score = log10f(a) - (kLog10Scale + log10f((float)beta));
                 raw = a / (float)beta;
             }
-            bd.scores[out] = score;
+            bd.scores[it.out] = score;
             if (p.rescue_on && !(raw >= p.rescue_min)) {
                 const int n = atomicAdd(p.rescue_counts + bin, 1);
-                p.rescue_list[begin + n] = idx;
+                p.rescue_items[rescue_base + n] = it;
             }
         } else {
             const double a = (double)acc;
-            bd.scores[out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
+            bd.scores[it.out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
             // one scale for the whole alignment: accept sums that stay far
             // above double's floor (score above ~ -572), where no cell that
             // matters can have been flushed; the rest go to the
             // exponent-tracking wave kernel
             if (!(a >= 0x1p-900 && a < 1e300)) {
                 const int n = atomicAdd(p.final_count, 1);
-                p.final_list[n] = idx;
+                p.final_items[n] = it;
             }
         }
     }
 }
 
-// Blocks loop over the tasks of one bin, so a fixed grid covers any count.
-// begin/count come from device memory when the pointers are non-null.
+// A task of 64/G consecutive alignments of an item list (the FP64 recompute
+// of rescue lists): every group sets up its own read and sweeps its own hap.
 template <typename T, int NR, bool FAST>
-__global__ __launch_bounds__(64) void k_fwd_tile(FwdParams p, const int32_t* list,
-                                                 const int32_t* begin_ptr,
-                                                 const int32_t* count_ptr, int begin, int count,
-                                                 int G, int bin)
+__device__ __forceinline__ void tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
+                                                const WorkItem* items, int begin, int count, int G,
+                                                int task, int rescue_base, int bin)
 {
-    static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
-    __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
-    if (begin_ptr)
-        begin = *begin_ptr;
-    if (count_ptr)
-        count = *count_ptr;
+    const BatchDesc& bd = *p.bd;
+    const int lane = threadIdx.x;
     const int W = 64 / G;
-    const int ntasks = (count + W - 1) / W;
-    for (int task = blockIdx.x; task < ntasks; task += gridDim.x)
-        fwd_tile_task<T, NR, FAST>(p, tab, list, begin, count, G, task, bin);
+    const int grp = lane / G;
+    const int t = lane - grp * G;
+    const int local = task * W + grp;
+    const bool active = grp < W && local < count;
+    WorkItem it;
+    it.read = 0;
+    it.hap = 0;
+    it.out = -1;
+    it.pad = 0;
+    if (active)
+        it = items[begin + local];
+    TileRead<T, NR> R;
+    tile_setup<T, NR, FAST>(p, tab, active, it.read, t, R);
+    const int HL = active ? bd.hap_len[it.hap] : 0;
+    const int hoff = active ? bd.hap_off[it.hap] : 0;
+    tile_sweep<T, NR, FAST>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
+}
+
+// A super-task: the groups set up their reads once and sweep the segment's
+// haps in turn.  A chunk of nc < 64/G reads (a record's last) spreads over
+// rep = (64/G)/nc copies of itself: copy q sweeps haps hs+q, hs+q+rep, ...,
+// so the groups stay busy.  Empty haps were scored -inf by the scatter;
+// under reference CPU semantics, haps holding a byte other than A/C/G/T/N
+// belong to the exact bin.
+template <typename T, int NR, bool FAST>
+__device__ __forceinline__ void tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
+                                                const SuperTask& st, int G, int rescue_base, int bin)
+{
+    const BatchDesc& bd = *p.bd;
+    const int lane = threadIdx.x;
+    const int W = 64 / G;
+    const int grp = lane / G;
+    const int t = lane - grp * G;
+    const int rep = W / st.nc;  // 1 <= nc <= W
+    const int q = grp / st.nc;
+    const bool active = grp < W && q < rep;
+    const int rd = active ? p.read_perm[st.perm + (grp - q * st.nc)] : 0;
+    TileRead<T, NR> R;
+    tile_setup<T, NR, FAST>(p, tab, active, rd, t, R);
+    for (int hb = st.hs; hb < st.he; hb += rep) {  // same trip count on every lane
+        const int h = hb + q;
+        const int hap = st.h0 + min(h, st.he - 1);
+        const int HL = bd.hap_len[hap];
+        const bool go = active && h < st.he && HL > 0 && !(p.semantics == 1 && p.hapflag[hap]);
+        if (__ballot(go) == 0ull)
+            continue;
+        WorkItem it;
+        it.read = rd;
+        it.hap = hap;
+        it.out = st.aoff + (rd - st.r0) * st.nh + h;
+        it.pad = 0;
+        tile_sweep<T, NR, FAST>(p, tab, R, go, G, t, go ? HL : 0, go ? bd.hap_off[hap] : 0, it,
+                                rescue_base, bin);
+    }
 }
 
 // Lane i < bs.n of a wave describes bin i of a BinSet: its id, lanes per
@@ -477,7 +543,7 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
     static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
     __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
     SetLane sl;
-    const int total = set_lanes(bs, 0, sl);
+    const int total = set_lanes(bs, bs.st ? 1 : 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
         return;
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
@@ -487,7 +553,12 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
         const int begin = __shfl(sl.begin, k, 64);
         const int count = __shfl(sl.count, k, 64);
         const int t0 = __shfl(sl.incl - sl.ntask, k, 64);
-        fwd_tile_task<T, NR, FAST>(p, tab, bs.list, begin, count, G, task - t0, bin);
+        const int rescue_base = bs.seg ? bs.seg[2 * bin] : 0;
+        if (bs.st)
+            tile_super_task<T, NR, FAST>(p, tab, bs.st[begin + (task - t0)], G, rescue_base, bin);
+        else
+            tile_items_task<T, NR, FAST>(p, tab, bs.items, begin, count, G, task - t0, rescue_base,
+                                         bin);
     }
 }
 
@@ -558,8 +629,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
         const int kb = set_find(sl, w);
         const int begin = __shfl(sl.begin, kb, 64);
         const int w0 = __shfl(sl.incl - sl.ntask, kb, 64);
-        const int idx = bs.list ? bs.list[begin + (w - w0)] : begin + (w - w0);
-        const WorkItem it = p.items[idx];
+        const WorkItem it = bs.items[begin + (w - w0)];
         const int L = bd.read_len[it.read];
         const int HL = bd.hap_len[it.hap];
         const int roff = bd.read_off[it.read];
@@ -743,7 +813,7 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
             const double lmin = log10((double)p.rescue_min) - (double)kLog10Scale;
             if (sizeof(T) == 4 && p.rescue_on && !(acc > (T)0 && lg >= lmin && lg < 1e30)) {
                 const int q = atomicAdd(p.final_count, 1);
-                p.final_list[q] = idx;
+                p.final_items[q] = it;
             }
         }
     }
@@ -752,48 +822,6 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T, int NR>
-static hipError_t launch_tile_nr(bool fast, const FwdParams& p, const TileLaunch& tl,
-                                 hipStream_t st)
-{
-    int blocks = tl.max_blocks;
-    if (!tl.count_ptr) {
-        const int W = 64 / tl.G;
-        const int need = (tl.count + W - 1) / W;
-        if (need <= 0)
-            return hipSuccess;
-        blocks = tl.max_blocks > 0 && tl.max_blocks < need ? tl.max_blocks : need;
-    }
-    if (blocks <= 0)
-        return hipErrorInvalidValue;
-    const dim3 grid(blocks), block(64);
-    if (fast && sizeof(T) == 4)
-        hipLaunchKernelGGL((k_fwd_tile<float, NR, true>), grid, block, 0, st, p, tl.list,
-                           tl.begin_ptr, tl.count_ptr, tl.begin, tl.count, tl.G, tl.bin);
-    else
-        hipLaunchKernelGGL((k_fwd_tile<T, NR, false>), grid, block, 0, st, p, tl.list,
-                           tl.begin_ptr, tl.count_ptr, tl.begin, tl.count, tl.G, tl.bin);
-    return hipGetLastError();
-}
-
-hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaunch& tl,
-                           hipStream_t st)
-{
-    if (tl.G < 1 || tl.G > 64)
-        return hipErrorInvalidValue;
-    switch (NR) {
-    case 4: return launch_tile_nr<float, 4>(fast, p, tl, st);
-    case 8: return launch_tile_nr<float, 8>(fast, p, tl, st);
-    case 12: return launch_tile_nr<float, 12>(fast, p, tl, st);
-    case 16: return launch_tile_nr<float, 16>(fast, p, tl, st);
-    case 20: return launch_tile_nr<float, 20>(fast, p, tl, st);
-    case 24: return launch_tile_nr<float, 24>(fast, p, tl, st);
-    case 28: return launch_tile_nr<float, 28>(fast, p, tl, st);
-    case 32: return launch_tile_nr<float, 32>(fast, p, tl, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 template <typename K>
 static int resident_blocks_per_cu(K kernel)
 {
@@ -834,7 +862,7 @@ template <typename T, int NR>
 static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                 hipStream_t st)
 {
-    if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.next_task)
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.next_task || (!bs.items && !bs.st))
         return hipErrorInvalidValue;
     for (int i = 0; i < bs.n; ++i)
         if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
@@ -898,7 +926,7 @@ hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32
                                float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st)
 {
-    if (bs.n < 1 || bs.n > 64 || !bs.counts || blocks <= 0)
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.items || blocks <= 0)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fwd_wave<float, 16>), dim3(blocks), dim3(64), 0, st, p, bs, hmax_ptr,
                        spill, spill_elems, error_flag);
@@ -909,7 +937,7 @@ hipError_t launch_fwd_wave_f64(const FwdParams& p, const BinSet& bs, const int32
                                double* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st)
 {
-    if (bs.n < 1 || bs.n > 64 || !bs.counts || blocks <= 0)
+    if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.items || blocks <= 0)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fwd_wave<double, 8>), dim3(blocks), dim3(64), 0, st, p, bs, hmax_ptr,
                        spill, spill_elems, error_flag);

@@ -15,6 +15,21 @@ struct __attribute__((aligned(16))) WorkItem {
     int32_t pad;
 };
 
+// Work of a tile bin: up to 64/lanes reads of one record (consecutive in
+// read_perm, all in the same bin) against a segment [hs, he) of the record's
+// haps.  One wave sets the reads up once and sweeps the segment's haps one
+// after another; segments hold at most a column budget (at least one hap),
+// so tasks stay numerous enough to balance.
+struct __attribute__((aligned(16))) SuperTask {
+    int32_t perm;    // index in read_perm of the first read
+    int32_t nc;      // reads in the task (groups of the wave in use)
+    int32_t h0;      // first hap of the record (global index)
+    int32_t nh;      // haps of the record
+    int32_t hs, he;  // the segment: record haps hs .. he-1
+    int32_t r0;      // first read of the record (global index)
+    int32_t aoff;    // output index of the record's first alignment
+};
+
 // Scale of the forward variables: the reference starts row 0's deletion
 // state at FLT_MAX/16 / H and returns log10(sum) - log10(FLT_MAX/16)
 // (pairhmm_kernels.cuh:1203, 1673, 1760-1763).
@@ -36,7 +51,7 @@ constexpr int kMaxTiles = 32;
 // Bins: 2 per tile class (FAST, SAFE), the multi-tile bin, and the exact
 // bin (reference CPU semantics, bytes other than A/C/G/T/N present).
 constexpr int kMaxBins = 2 * kMaxTiles + 2;
-constexpr int kTaskCounters = 32;   // set launches per call
+constexpr int kTaskCounters = 64;   // set launches per call (plain mode: one per bin)
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
@@ -102,21 +117,23 @@ struct BatchDesc {
     const int32_t* rprefix;    // [G+1]
     const int32_t* hprefix;    // [G+1]
     float* scores;
-    int32_t G, nR, nH, pad;
+    int32_t G, nR, nH;
+    int32_t col_budget;  // hap columns per super-task segment
     int64_t read_bytes, hap_bytes, num_alignments;
 };
 
 // Everything the forward kernels read besides the batch descriptor.
 struct FwdParams {
     const BatchDesc* bd;
-    const WorkItem* items;
     const uint32_t* rows;        // packed read rows, indexed like read bytes
     const uint8_t* hapc;         // hap base codes, indexed like hap bytes
     const ReadMeta* meta;
+    const int32_t* read_perm;    // reads of each record ordered by bin
+    const uint8_t* hapflag;      // CPU semantics: hap holds a byte other than A/C/G/T/N
     const float* ph2pr;          // 128 entries
-    int32_t* rescue_list;        // per bin: work-item indices to recompute in FP64
+    WorkItem* rescue_items;      // per tile bin: alignments to recompute in FP64
     int32_t* rescue_counts;      // per bin
-    int32_t* final_list;         // items the FP64 tile pass could not finish
+    WorkItem* final_items;       // alignments the FP64 tile pass could not finish
     int32_t* final_count;
     float rescue_min;            // FP32 sum (reference units) threshold
     int32_t rescue_on;
@@ -138,18 +155,21 @@ struct BinParams {
     int64_t* align_off;        // [G+1]
     int32_t* key_count;        // [NB*kHapBuckets]
     int32_t* key_cursor;       // [NB*kHapBuckets]
-    int32_t* bin_range;        // [2*NB]: begin, count
+    int32_t* bin_range;        // [2*NB]: begin, count (tile bins: super-tasks; others: items)
     int32_t* hmax;             // [1]
     int32_t* rescue_counts;    // [NB]
     int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
     int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
+    SuperTask* st;             // tile bins' work, in bin_range units
+    int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
     uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T/N (CPU semantics)
     WorkItem* items;
     const float* ph2pr;
     // tiles
     int32_t ntiles;
     int32_t tiles[kMaxTiles];  // ascending max read length per tile class
+    int32_t lanes[kMaxTiles];  // lanes per alignment of each tile class
     int32_t semantics;
     int32_t allow_fast;
 };
@@ -159,33 +179,24 @@ struct BinParams {
 // capture), else grids are sized to them.
 hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, hipStream_t st);
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
-// One launch of k_fwd_tile over one bin (exact grid; plain mode).
-struct TileLaunch {
-    const int32_t* list = nullptr;       // item indices, or null: items are contiguous
-    const int32_t* begin_ptr = nullptr;  // device begin/count (null: host values)
-    const int32_t* count_ptr = nullptr;
-    int begin = 0, count = 0;
-    int G = 8;                           // lanes per alignment
-    int bin = 0;                         // bin id (rescue list slot)
-    int max_blocks = 0;                  // grid cap (device count: the grid)
-};
-hipError_t launch_fwd_tile(int NR, bool fast, const FwdParams& p, const TileLaunch& tl,
-                           hipStream_t st);
-
-// Several bins in one persistent launch: the work of bin bin[i] is
-// list ? list[begin_i + w] : begin_i + w for w < count_i, with
-// begin_i = range ? range[2*bin[i]] : 0 and count_i =
-// counts[count_stride*bin[i]], both read on the device.  The bins' tasks
-// form one sequence in the given order; waves take the next task from
-// *next_task (zeroed before the launch).
+// Several bins in one persistent launch.  Bin bin[i] holds count_i =
+// counts[count_stride*bin[i]] units starting at begin_i = range ?
+// range[2*bin[i]] : 0, both read on the device.  A unit is a SuperTask
+// (st != null: tile kernels, one task each) or a WorkItem of `items` (tile
+// kernels: 64/lanes per task; wave kernels: one each).  The bins' tasks form
+// one sequence in the given order; tile-kernel waves take the next task from
+// *next_task (zeroed before the launch).  seg[bin] is where a tile bin's
+// FP32 rescue list starts in FwdParams::rescue_items.
 struct BinSet {
     int32_t n = 0;
     int32_t bin[kMaxBins] = {};
     int32_t lanes[kMaxBins] = {};  // tile kernels: lanes per alignment
-    const int32_t* list = nullptr;
+    const WorkItem* items = nullptr;
+    const SuperTask* st = nullptr;
     const int32_t* range = nullptr;
     const int32_t* counts = nullptr;
     int32_t count_stride = 0;
+    const int32_t* seg = nullptr;
     int32_t* next_task = nullptr;
 };
 // grid: 0 = as many blocks as fit on the device at once
