@@ -369,10 +369,33 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
 }
 
 __device__ __forceinline__ bool is_tile_bin(const BinParams& bp, int k) { return k < 2 * bp.ntiles; }
-__device__ __forceinline__ int tasks_of(const BinParams& bp, int k, int reads)
+// A tile bin's `reads` reads of a group split into chunks of W = 64/lanes
+// reads: one super-task per full chunk and hap segment.  A short last chunk
+// of n = reads % W reads runs as rep = W/n copies that sweep rep haps at a
+// time, so it takes a run of rep consecutive segments per super-task (at
+// the run's first segment); its sort key is the run's columns per copy.
+struct ChunkPlan {
+    int W, nfull, rem, rep;
+};
+__device__ __forceinline__ ChunkPlan chunk_plan(const BinParams& bp, int k, int reads)
 {
-    const int W = 64 / bp.lanes[k >> 1];
-    return (reads + W - 1) / W;
+    ChunkPlan c;
+    c.W = 64 / bp.lanes[k >> 1];
+    c.nfull = reads / c.W;
+    c.rem = reads - c.nfull * c.W;
+    c.rep = c.rem > 0 ? c.W / c.rem : 1;
+    return c;
+}
+__device__ __forceinline__ bool rem_task_at(const ChunkPlan& c, int sg)
+{
+    return c.rem > 0 && sg % c.rep == 0;
+}
+__device__ __forceinline__ int rem_work(const GroupHaps& gh, const ChunkPlan& c, int sg)
+{
+    int w = 0;
+    for (int i = sg; i < min(sg + c.rep, gh.nseg); ++i)
+        w += gh.work[i];
+    return (w + c.rep - 1) / c.rep;
 }
 
 // Per group (one wave): the sort-key histogram and the per-bin alignment
@@ -409,8 +432,13 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
         for (int x = threadIdx.x; x < n * gh.nseg; x += 64) {
             const int e = x / gh.nseg, sg = x - e * gh.nseg;
             const int k = gb.k[e];
-            if (k != bp.NB && is_tile_bin(bp, k))
-                atomicAdd(&bp.key_count[key_of(k, gh.work[sg])], tasks_of(bp, k, gb.cnt[e]));
+            if (k != bp.NB && is_tile_bin(bp, k)) {
+                const ChunkPlan c = chunk_plan(bp, k, gb.cnt[e]);
+                if (c.nfull > 0)
+                    atomicAdd(&bp.key_count[key_of(k, gh.work[sg])], c.nfull);
+                if (rem_task_at(c, sg))
+                    atomicAdd(&bp.key_count[key_of(k, rem_work(gh, c, sg))], 1);
+            }
         }
         // per (hap, bin) item run of the one-wave-per-alignment bins
         for (int x = threadIdx.x; x < nh * n; x += 64) {
@@ -503,7 +531,8 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 }
 
 // Per group (one wave): write the work lists.  Tile bins: the group's reads
-// of the bin in chunks of 64/lanes, one super-task each (all haps).  The
+// of the bin in chunks of 64/lanes, one super-task per chunk and hap
+// segment (a short last chunk: per run of segments, as chunk_plan says).  The
 // multi-tile and exact bins: one item per alignment, reserved per (hap, bin)
 // run -- one lane per pair, haps in chunks that fit the LDS table -- and
 // written with lanes over the reads of a run.  Alignments with an empty read
@@ -530,24 +559,37 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
             const int k = gb.k[e];
             if (k == bp.NB || !is_tile_bin(bp, k))
                 continue;
-            const int c = gb.cnt[e], W = 64 / bp.lanes[k >> 1];
-            const int nt = (c + W - 1) / W;
+            const int c = gb.cnt[e];
+            const ChunkPlan cp = chunk_plan(bp, k, c);
+            SuperTask t;
+            t.h0 = h0;
+            t.nh = nh;
+            t.r0 = r0;
+            t.aoff = (int32_t)A;
             for (int sg = 0; sg < gh.nseg; ++sg) {
-                int base = 0;
-                if (lane == 0)
-                    base = bp.bin_range[2 * k] + atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], nt);
+                int base = 0, rbase = 0;
+                if (lane == 0) {
+                    if (cp.nfull > 0)
+                        base = bp.bin_range[2 * k] +
+                               atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], cp.nfull);
+                    if (rem_task_at(cp, sg))
+                        rbase = bp.bin_range[2 * k] +
+                                atomicAdd(&bp.key_cursor[key_of(k, rem_work(gh, cp, sg))], 1);
+                }
                 base = __shfl(base, 0, 64);
-                for (int j = lane; j < nt; j += 64) {
-                    SuperTask t;
-                    t.perm = r0 + gb.st[e] + j * W;
-                    t.nc = min(W, c - j * W);
-                    t.h0 = h0;
-                    t.nh = nh;
-                    t.hs = gh.hs[sg];
+                rbase = __shfl(rbase, 0, 64);
+                t.hs = gh.hs[sg];
+                for (int j = lane; j < cp.nfull; j += 64) {
+                    t.perm = r0 + gb.st[e] + j * cp.W;
+                    t.nc = cp.W;
                     t.he = gh.he[sg];
-                    t.r0 = r0;
-                    t.aoff = (int32_t)A;
                     bp.st[base + j] = t;
+                }
+                if (lane == 0 && rem_task_at(cp, sg)) {
+                    t.perm = r0 + gb.st[e] + cp.nfull * cp.W;
+                    t.nc = cp.rem;
+                    t.he = gh.he[min(sg + cp.rep, gh.nseg) - 1];
+                    bp.st[rbase] = t;
                 }
             }
         }
