@@ -255,8 +255,11 @@ __device__ __forceinline__ int key_of(int bin, int hl)
     return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
 }
 
-// Reference CPU semantics: per hap (one wave each), whether it holds a byte
-// other than A/C/G/T/N.
+// Per hap (one wave each), whether it holds a byte other than A/C/G/T: the
+// wildcard (reference GPU semantics: any such byte; CPU semantics: 'N') or,
+// under CPU semantics, a byte that matches nothing but itself.  Alignments
+// with such haps go to the exact bin, so the tile kernels' emission tables
+// need only the four bases.
 __global__ __launch_bounds__(256) void k_hap_flags(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
@@ -264,12 +267,12 @@ __global__ __launch_bounds__(256) void k_hap_flags(BinParams bp)
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     for (int h = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); h < b.nH; h += nwaves) {
         const int hl = b.hap_len[h], off = b.hap_off[h];
-        bool exotic = false;
+        bool other = false;
         for (int c = lane; c < hl; c += 64)
-            exotic = exotic || bp.hapc[off + c] == kExoticCode;
-        exotic = __any(exotic);
+            other = other || bp.hapc[off + c] >= 4;
+        other = __any(other);
         if (lane == 0)
-            bp.hapflag[h] = exotic ? 1 : 0;
+            bp.hapflag[h] = other ? 1 : 0;
     }
 }
 
@@ -306,12 +309,12 @@ __device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, G
 }
 
 // Per-group hap summary, uniform across the wave: haps with a sequence and
-// those of them that go to the exact bin (CPU semantics); the others are
+// those of them that go to the exact bin (a byte other than A/C/G/T); the others are
 // cut, in order, into segments of at most `budget` columns (at least one hap
 // each) -- the super-tasks' hap ranges and their work (sort key).
 constexpr int kMaxSegs = 512;
 struct GroupHaps {
-    int nonempty, exotic, nseg;
+    int nonempty, other, nseg;
     int hs[kMaxSegs], he[kMaxSegs], work[kMaxSegs];
 };
 __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc& b, int h0, int nh,
@@ -322,7 +325,7 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
         const int hl = b.hap_len[h0 + h];
         if (hl > 0) {
             ++ne;
-            if (bp.semantics == 1 && bp.hapflag[h0 + h])
+            if (bp.hapflag[h0 + h] != 0)
                 ++ex;
             else
                 tot += hl;
@@ -339,11 +342,11 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
     const int budget = max(b.col_budget, (2 * tot + kMaxSegs - 2) / (kMaxSegs - 1));
     if (threadIdx.x == 0) {
         gh.nonempty = ne;
-        gh.exotic = ex;
+        gh.other = ex;
         int n = 0, cur = 0, start = 0;
         for (int h = 0; h < nh; ++h) {
             const int hl = b.hap_len[h0 + h];
-            if (hl <= 0 || (bp.semantics == 1 && bp.hapflag[h0 + h]))
+            if (hl <= 0 || (bp.hapflag[h0 + h] != 0))
                 continue;
             if (cur > 0 && cur + hl > budget) {
                 gh.hs[n] = start;
@@ -419,11 +422,11 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             if (k == bp.NB)
                 continue;
             if (is_tile_bin(bp, k)) {
-                const int tiled = gh.nonempty - gh.exotic;
+                const int tiled = gh.nonempty - gh.other;
                 if (tiled > 0)
                     atomicAdd(&bp.align_range[2 * k + 1], c * tiled);
-                if (gh.exotic > 0)
-                    atomicAdd(&bp.align_range[2 * eb + 1], c * gh.exotic);
+                if (gh.other > 0)
+                    atomicAdd(&bp.align_range[2 * eb + 1], c * gh.other);
             } else if (gh.nonempty > 0) {
                 atomicAdd(&bp.align_range[2 * k + 1], c * gh.nonempty);
             }
@@ -447,8 +450,8 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             const int hl = b.hap_len[h0 + h];
             if (hl <= 0 || k == bp.NB)
                 continue;
-            const bool exotic = bp.semantics == 1 && bp.hapflag[h0 + h];
-            if (is_tile_bin(bp, k) && !exotic)
+            const bool other = bp.hapflag[h0 + h] != 0;
+            if (is_tile_bin(bp, k) && !other)
                 continue;  // a super-task covers it
             atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl)], gb.cnt[e]);
         }
@@ -603,8 +606,8 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                 const int hl = b.hap_len[h0 + h];
                 int base = -1;  // -inf
                 if (hl > 0 && k != bp.NB) {
-                    const bool exotic = bp.semantics == 1 && bp.hapflag[h0 + h];
-                    if (is_tile_bin(bp, k) && !exotic) {
+                    const bool other = bp.hapflag[h0 + h] != 0;
+                    if (is_tile_bin(bp, k) && !other) {
                         base = kSkip;
                     } else {
                         const int kb = is_tile_bin(bp, k) ? eb : k;
@@ -685,7 +688,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.nR > 0)  // 16 lanes per read
         hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 16, 4096)), dim3(tpb), 0, st, bp);
-    if (bp.semantics == 1 && (fixed_grids || hd.nH > 0))  // one wave per hap
+    if (fixed_grids || hd.nH > 0)  // one wave per hap
         hipLaunchKernelGGL(k_hap_flags, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {

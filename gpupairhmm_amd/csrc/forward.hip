@@ -13,8 +13,9 @@
 // the haplotype streams through the group one column per step (anti-diagonal
 // wavefront).  Row-above values cross lanes with DPP wave_shr:1, a group's
 // first lane injects the row-0 boundary.  Emission probabilities for the
-// 5 hap letters x NR rows of each lane sit in LDS (one ds_read_b128 per
-// 4 cells, conflict-free by lane interleaving).  Two arithmetic forms:
+// 4 bases x NR rows of each lane sit in LDS (one ds_read_b128 per 4 cells,
+// conflict-free by lane interleaving; haps holding any other byte go to the
+// one-wave kernel).  Two arithmetic forms:
 //
 //  SAFE (6 VALU/cell): variables M, Ib = beta*I, Dh = D/sigma_r.
 //    M  = lambda*(alpha M' + Ib' + beta sigma_{r-1} Dh')      fma fma mul
@@ -228,15 +229,17 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
             }
         }
         // emission table: tab[(letter*NQ + q)*64 + lane] = rows K*q .. K*q+K-1
+        // for the four bases (haps with any other byte go to the one-wave
+        // kernel; a read's 'N' matches every base)
 #pragma unroll
-        for (int l = 0; l < 5; ++l) {
+        for (int l = 0; l < 4; ++l) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 T v[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int j = K * q + k;
-                    const bool m = (l == code[j]) || (l == 4) || (code[j] == 4);
+                    const bool m = (l == code[j]) || (code[j] == 4);
                     v[k] = m ? lm[j] : lx[j];
                 }
                 tab[(l * NQ + q) * 64 + lane] = TT::pack(v);
@@ -324,7 +327,7 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     V lam[2][NQ];
     uint32_t letters = __builtin_amdgcn_alignbyte(whi, wlo, shift);
     {
-        const V* trow = tab + __umul24(letters & 0xffu, NQ * 64) + lane_off;
+        const V* trow = tab + __umul24(letters & 3u, NQ * 64) + lane_off;
 #pragma unroll
         for (int qq = 0; qq < NQ; ++qq)
             lam[0][qq] = trow[qq * 64];
@@ -336,7 +339,9 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int cur = u & 1, nxt = cur ^ 1;
-            const uint32_t nletter = u < 3 ? (letters >> (8 * (u + 1))) & 0xffu : next & 0xffu;
+            // codes are 0..3 inside the hap; the padding's wildcard code
+            // reads as base 0 there, and padding columns are never accumulated
+            const uint32_t nletter = u < 3 ? (letters >> (8 * (u + 1))) & 3u : next & 3u;
             {
                 const V* trow = tab + __umul24(nletter, NQ * 64) + lane_off;
 #pragma unroll
@@ -471,7 +476,7 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
         const int h = hb + q;
         const int hap = st.h0 + min(h, st.he - 1);
         const int HL = bd.hap_len[hap];
-        const bool go = active && h < st.he && HL > 0 && !(p.semantics == 1 && p.hapflag[hap]);
+        const bool go = active && h < st.he && HL > 0 && p.hapflag[hap] == 0;
         if (__ballot(go) == 0ull)
             continue;
         WorkItem it;
@@ -541,7 +546,7 @@ template <typename T, int NR, bool FAST>
 __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
 {
     static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
-    __shared__ typename TileTraits<T>::V tab[5 * (NR / TileTraits<T>::K) * 64];
+    __shared__ typename TileTraits<T>::V tab[4 * (NR / TileTraits<T>::K) * 64];
     SetLane sl;
     const int total = set_lanes(bs, bs.st ? 1 : 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
