@@ -316,7 +316,23 @@ constexpr int kMaxSegs = 512;
 struct GroupHaps {
     int nonempty, other, nseg;
     int hs[kMaxSegs], he[kMaxSegs], work[kMaxSegs];
+    int cols[64];         // staging: columns of 64 haps (0: not cut into segments)
+    int info[kMaxSegs];   // haps < kMaxSegs: length (0 if empty), top bit: flagged
 };
+// Length (<= 0: empty) and flag of record hap h, from the LDS copy where
+// group_haps made one.
+__device__ __forceinline__ void hap_info(const BinParams& bp, const BatchDesc& b,
+                                         const GroupHaps& gh, int h0, int h, int& hl, bool& flag)
+{
+    if (h < kMaxSegs) {
+        const int v = gh.info[h];
+        hl = v & 0x7fffffff;
+        flag = v < 0;
+    } else {
+        hl = b.hap_len[h0 + h];
+        flag = bp.hapflag[h0 + h] != 0;
+    }
+}
 __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc& b, int h0, int nh,
                                            GroupHaps& gh)
 {
@@ -340,26 +356,43 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
     // greedy segments number at most 2*tot/budget + 1: a larger budget where
     // they would not fit the table
     const int budget = max(b.col_budget, (2 * tot + kMaxSegs - 2) / (kMaxSegs - 1));
+    // lane 0 cuts greedily over haps staged 64 at a time in LDS (one
+    // parallel load per chunk instead of a serial chain of global loads)
+    int n = 0, cur = 0, start = 0;
+    for (int c0 = 0; c0 < nh; c0 += 64) {
+        const int h = c0 + (int)threadIdx.x;
+        int cols = 0;
+        if (h < nh) {
+            const int hl = b.hap_len[h0 + h];
+            const bool flag = hl > 0 && bp.hapflag[h0 + h] != 0;
+            cols = (hl > 0 && !flag) ? hl : 0;
+            if (h < kMaxSegs)
+                gh.info[h] = (hl > 0 ? hl : 0) | (flag ? (int)0x80000000u : 0);
+        }
+        gh.cols[threadIdx.x] = cols;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < min(64, nh - c0); ++i) {
+                const int hl = gh.cols[i];
+                if (hl == 0)
+                    continue;
+                if (cur > 0 && cur + hl > budget) {
+                    gh.hs[n] = start;
+                    gh.he[n] = c0 + i;
+                    gh.work[n] = cur;
+                    ++n;
+                    cur = 0;
+                }
+                if (cur == 0)
+                    start = c0 + i;
+                cur += hl;
+            }
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         gh.nonempty = ne;
         gh.other = ex;
-        int n = 0, cur = 0, start = 0;
-        for (int h = 0; h < nh; ++h) {
-            const int hl = b.hap_len[h0 + h];
-            if (hl <= 0 || (bp.hapflag[h0 + h] != 0))
-                continue;
-            if (cur > 0 && cur + hl > budget) {
-                gh.hs[n] = start;
-                gh.he[n] = h;
-                gh.work[n] = cur;
-                ++n;
-                cur = 0;
-                start = h;
-            }
-            if (cur == 0)
-                start = h;
-            cur += hl;
-        }
         if (cur > 0) {
             gh.hs[n] = start;
             gh.he[n] = nh;
@@ -416,7 +449,10 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
-        // per bin entry: alignment counts
+        // per bin entry: alignment counts, spread over kAlignParts counters
+        // per bin (every group adds to its bins: one counter per bin would
+        // serialize the atomics of all groups)
+        const int part = g & (kAlignParts - 1);
         for (int e = threadIdx.x; e < n; e += 64) {
             const int k = gb.k[e], c = gb.cnt[e];
             if (k == bp.NB)
@@ -424,11 +460,11 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             if (is_tile_bin(bp, k)) {
                 const int tiled = gh.nonempty - gh.other;
                 if (tiled > 0)
-                    atomicAdd(&bp.align_range[2 * k + 1], c * tiled);
+                    atomicAdd(&bp.align_part[k * kAlignParts + part], c * tiled);
                 if (gh.other > 0)
-                    atomicAdd(&bp.align_range[2 * eb + 1], c * gh.other);
+                    atomicAdd(&bp.align_part[eb * kAlignParts + part], c * gh.other);
             } else if (gh.nonempty > 0) {
-                atomicAdd(&bp.align_range[2 * k + 1], c * gh.nonempty);
+                atomicAdd(&bp.align_part[k * kAlignParts + part], c * gh.nonempty);
             }
         }
         // per (tile bin entry, hap segment): its super-tasks' key
@@ -447,10 +483,11 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
         for (int x = threadIdx.x; x < nh * n; x += 64) {
             const int h = x / n, e = x - h * n;
             const int k = gb.k[e];
-            const int hl = b.hap_len[h0 + h];
+            int hl;
+            bool other;
+            hap_info(bp, b, gh, h0, h, hl, other);
             if (hl <= 0 || k == bp.NB)
                 continue;
-            const bool other = bp.hapflag[h0 + h] != 0;
             if (is_tile_bin(bp, k) && !other)
                 continue;  // a super-task covers it
             atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl)], gb.cnt[e]);
@@ -509,28 +546,41 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
 
 // Bin begins: exclusive scans of the bin counts -- in work units
 // (bin_range) and in alignments (align_range) -- one wave, NB <= 66.
-__device__ __forceinline__ void scan_pairs(int32_t* range, int NB)
+// Counts come from range[2k+1], or from cnt[k] (then also stored there).
+__device__ __forceinline__ void scan_pairs(int32_t* range, int NB, const int* cnt)
 {
     int carry = 0;
     for (int base = 0; base < NB; base += 64) {
         const int k = base + (int)threadIdx.x;
-        const int v = k < NB ? range[2 * k + 1] : 0;
+        const int v = k < NB ? (cnt ? cnt[k] : range[2 * k + 1]) : 0;
         int x = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o, 64);
             if ((int)threadIdx.x >= o)
                 x += y;
         }
-        if (k < NB)
+        if (k < NB) {
             range[2 * k] = carry + x - v;
+            if (cnt)
+                range[2 * k + 1] = v;
+        }
         carry += __shfl(x, 63, 64);
     }
 }
 
 __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 {
-    scan_pairs(bp.bin_range, bp.NB);
-    scan_pairs(bp.align_range, bp.NB);
+    __shared__ int tot[kMaxBins];
+    scan_pairs(bp.bin_range, bp.NB, nullptr);
+    for (int k = threadIdx.x; k < bp.NB; k += 64) {  // each bin's alignment count from its parts
+        int v = 0;
+#pragma unroll 16
+        for (int q = 0; q < kAlignParts; ++q)
+            v += bp.align_part[k * kAlignParts + q];
+        tot[k] = v;
+    }
+    __syncthreads();
+    scan_pairs(bp.align_range, bp.NB, tot);
 }
 
 // Per group (one wave): write the work lists.  Tile bins: the group's reads
@@ -543,11 +593,11 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    constexpr int kRuns = 2048;
+    constexpr int kRuns = 1024;
     constexpr int kSkip = -2;  // run covered by a super-task
     __shared__ GroupBins gb;
     __shared__ GroupHaps gh;
-    __shared__ int run_base[kRuns];
+    __shared__ int run_x[kRuns], run_base[kRuns];
     const int lane = threadIdx.x;
     const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
@@ -557,71 +607,81 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
         const int64_t A = bp.align_off[g];
         group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
-        // super-tasks of the tile bins: read chunks x hap segments
-        for (int e = 0; e < n; ++e) {
+        // super-tasks of the tile bins: read chunks x hap segments, one lane
+        // per (bin entry, segment) pair: it reserves the pair's list slots
+        // and writes its tasks
+        const int npairs = n * gh.nseg;
+        for (int x = lane; x < npairs; x += 64) {
+            const int e = x / gh.nseg, sg = x - e * gh.nseg;
             const int k = gb.k[e];
             if (k == bp.NB || !is_tile_bin(bp, k))
                 continue;
-            const int c = gb.cnt[e];
-            const ChunkPlan cp = chunk_plan(bp, k, c);
+            const ChunkPlan cp = chunk_plan(bp, k, gb.cnt[e]);
             SuperTask t;
             t.h0 = h0;
             t.nh = nh;
+            t.hs = gh.hs[sg];
             t.r0 = r0;
             t.aoff = (int32_t)A;
-            for (int sg = 0; sg < gh.nseg; ++sg) {
-                int base = 0, rbase = 0;
-                if (lane == 0) {
-                    if (cp.nfull > 0)
-                        base = bp.bin_range[2 * k] +
-                               atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], cp.nfull);
-                    if (rem_task_at(cp, sg))
-                        rbase = bp.bin_range[2 * k] +
-                                atomicAdd(&bp.key_cursor[key_of(k, rem_work(gh, cp, sg))], 1);
-                }
-                base = __shfl(base, 0, 64);
-                rbase = __shfl(rbase, 0, 64);
-                t.hs = gh.hs[sg];
-                for (int j = lane; j < cp.nfull; j += 64) {
+            if (cp.nfull > 0) {
+                const int base = bp.bin_range[2 * k] +
+                                 atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], cp.nfull);
+                t.nc = cp.W;
+                t.he = gh.he[sg];
+                for (int j = 0; j < cp.nfull; ++j) {
                     t.perm = r0 + gb.st[e] + j * cp.W;
-                    t.nc = cp.W;
-                    t.he = gh.he[sg];
                     bp.st[base + j] = t;
                 }
-                if (lane == 0 && rem_task_at(cp, sg)) {
-                    t.perm = r0 + gb.st[e] + cp.nfull * cp.W;
-                    t.nc = cp.rem;
-                    t.he = gh.he[min(sg + cp.rep, gh.nseg) - 1];
-                    bp.st[rbase] = t;
-                }
+            }
+            if (rem_task_at(cp, sg)) {
+                const int rbase = bp.bin_range[2 * k] +
+                                  atomicAdd(&bp.key_cursor[key_of(k, rem_work(gh, cp, sg))], 1);
+                t.perm = r0 + gb.st[e] + cp.nfull * cp.W;
+                t.nc = cp.rem;
+                t.he = gh.he[min(sg + cp.rep, gh.nseg) - 1];
+                bp.st[rbase] = t;
             }
         }
-        // item runs and -inf
+        // item runs and -inf: lanes reserve the runs of (hap, bin entry)
+        // pairs -- tile-bin pairs are covered by super-tasks -- and list the
+        // others compactly; then the wave writes each listed run, lanes over
+        // the run's reads
         const int hchunk = n > 0 ? max(1, kRuns / n) : nh;
         for (int hc = 0; hc < nh; hc += hchunk) {
             const int hcn = min(hchunk, nh - hc);
-            for (int x = lane; x < hcn * n; x += 64) {
-                const int h = hc + x / n, e = x % n;
-                const int k = gb.k[e];
-                const int hl = b.hap_len[h0 + h];
-                int base = -1;  // -inf
-                if (hl > 0 && k != bp.NB) {
-                    const bool other = bp.hapflag[h0 + h] != 0;
-                    if (is_tile_bin(bp, k) && !other) {
-                        base = kSkip;
-                    } else {
-                        const int kb = is_tile_bin(bp, k) ? eb : k;
-                        base = bp.bin_range[2 * kb] +
-                               atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
+            int nrun = 0;  // uniform
+            for (int x0 = 0; x0 < hcn * n; x0 += 64) {  // same trip count on every lane
+                const int x = x0 + lane;
+                int base = kSkip;
+                if (x < hcn * n) {
+                    const int h = hc + x / n, e = x % n;
+                    const int k = gb.k[e];
+                    int hl;
+                    bool other;
+                    hap_info(bp, b, gh, h0, h, hl, other);
+                    base = -1;  // -inf
+                    if (hl > 0 && k != bp.NB) {
+                        if (is_tile_bin(bp, k) && !other) {
+                            base = kSkip;
+                        } else {
+                            const int kb = is_tile_bin(bp, k) ? eb : k;
+                            base = bp.bin_range[2 * kb] +
+                                   atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
+                        }
                     }
                 }
-                run_base[x] = base;
+                const bool keep = base != kSkip;
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int pos = nrun + __popcll(m & ((1ull << lane) - 1ull));
+                    run_x[pos] = x;
+                    run_base[pos] = base;
+                }
+                nrun += __popcll(m);
             }
             __syncthreads();
-            for (int x = 0; x < hcn * n; ++x) {
-                const int base = run_base[x];
-                if (base == kSkip)
-                    continue;
+            for (int q = 0; q < nrun; ++q) {
+                const int x = run_x[q], base = run_base[q];
                 const int h = hc + x / n, e = x % n;
                 const int cnt = gb.cnt[e], st = gb.st[e];
                 for (int j = lane; j < cnt; j += 64) {
@@ -679,7 +739,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.task_ctr, (int64_t)kTaskCounters);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(256), 0, st, bp.align_range, (int64_t)2 * bp.NB);
+    hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, st, bp.align_part, (int64_t)kAlignParts * bp.NB);
     if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
         hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.hap_bytes > 0)

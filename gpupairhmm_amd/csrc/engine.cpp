@@ -214,7 +214,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NB, "bin ranges")) ||
         (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
-        (rc = ensure(align_range, sizeof(int32_t) * 2 * (size_t)NB, "alignment ranges")) ||
+        (rc = ensure(align_range, sizeof(int32_t) * (2 + kAlignParts) * (size_t)NB,
+                     "alignment ranges")) ||
         // each super-task covers at least one (read, hap) alignment and no
         // two cover the same one: at most num_alignments of them
         (rc = ensure(supertasks, sizeof(SuperTask) * (size_t)std::max<int64_t>(nA, 1), "super-tasks")) ||
@@ -310,6 +311,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.items = static_cast<WorkItem*>(items.p);
     bp.st = static_cast<SuperTask*>(supertasks.p);
     bp.align_range = static_cast<int32_t*>(align_range.p);
+    bp.align_part = bp.align_range + 2 * NB;
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
     for (int i = 0; i < ntiles; ++i) {

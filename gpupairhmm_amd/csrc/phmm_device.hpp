@@ -53,6 +53,7 @@ constexpr int kMaxTiles = 32;
 constexpr int kMaxBins = 2 * kMaxTiles + 2;
 constexpr int kTaskCounters = 64;   // set launches per call (plain mode: one per bin)
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
+constexpr int kAlignParts = 64;     // partial counters per bin (atomic spreading)
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
 __host__ __device__ inline int hap_bucket(int h)
@@ -163,6 +164,7 @@ struct BinParams {
     int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
     SuperTask* st;             // tile bins' work, in bin_range units
     int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
+    int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin
     uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T/N (CPU semantics)
     WorkItem* items;
     const float* ph2pr;
