@@ -453,7 +453,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             bs.bin[0] = b;
             bs.lanes[0] = tc.lanes;
             tile_set(bs);
-            if (ctr >= kTaskCounters - 2)
+            if (ctr >= kTaskCounters - 4)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             const int grid =
@@ -480,7 +480,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                     ++bs.n;
                 }
             tile_set(bs);
-            if (ctr >= kTaskCounters - 2)
+            if (ctr >= kTaskCounters - 4)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st), "forward kernel")))
@@ -520,8 +520,10 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
     const int32_t* h_align = h_bin_range + 2 * kMaxBins;
     int rc;
-    BinSet set4, set8, wave;
-    for (BinSet* x : {&set4, &set8, &wave}) {
+    // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
+    // the bin's form: FAST-eligible reads recompute in the FAST form)
+    BinSet set[2][2], wave;  // [NR 4, 8][SAFE, FAST]
+    for (BinSet* x : {&set[0][0], &set[0][1], &set[1][0], &set[1][1], &wave}) {
         x->items = fp.rescue_items;
         x->range = bp.align_range;
         x->counts = fp.rescue_counts;
@@ -531,19 +533,25 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
         if (plain && h_align[2 * b + 1] <= 0)
             continue;
         const int tile = tiles[(size_t)(b / 2)].max_read_len + 1;  // rows the FP64 tile must hold
-        BinSet& x = tile > 512 ? wave : tile <= 64 ? set4 : set8;
         const int nr64 = tile <= 64 ? 4 : 8;
+        BinSet& x = tile > 512 ? wave : set[nr64 == 8][b % 2 == 0];
         x.bin[x.n] = b;
         x.lanes[x.n] = tile > 512 ? 64 : (tile + nr64 - 1) / nr64;
         ++x.n;
     }
-    set8.next_task = bp.task_ctr + (kTaskCounters - 2);
-    set4.next_task = bp.task_ctr + (kTaskCounters - 1);
-    if ((set8.n && (rc = check(launch_fwd_tile_set_f64(8, fp, set8, 0, st), "FP64 tile kernel"))) ||
-        (set4.n && (rc = check(launch_fwd_tile_set_f64(4, fp, set4, 0, st), "FP64 tile kernel"))) ||
-        (wave.n && (rc = check(launch_fwd_wave_f64(fp, wave, bp.hmax, spill64, n64, bp.error_flag,
-                                                   num_cus * 8, st),
-                               "FP64 wave kernel"))))
+    for (int i = 0; i < 2; ++i)
+        for (int f = 0; f < 2; ++f) {
+            BinSet& x = set[1 - i][f];  // 8 rows per lane first
+            if (!x.n)
+                continue;
+            x.next_task = bp.task_ctr + (kTaskCounters - 4 + 2 * i + f);
+            if ((rc = check(launch_fwd_tile_set_f64(i == 0 ? 8 : 4, f == 1, fp, x, 0, st),
+                            "FP64 tile kernel")))
+                return rc;
+        }
+    if (wave.n && (rc = check(launch_fwd_wave_f64(fp, wave, bp.hmax, spill64, n64, bp.error_flag,
+                                                  num_cus * 8, st),
+                              "FP64 wave kernel")))
         return rc;
     BinSet fin;
     fin.n = 1;

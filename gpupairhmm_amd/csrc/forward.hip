@@ -286,7 +286,8 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
         if (F32)
             C0 = FAST ? (T)((kScale / (float)HL) * (nl * (1.0f / kFastKappa))) : (T)(kScale / (float)HL);
         else
-            C0 = (T)(kStartF64 / (double)HL);
+            C0 = FAST ? (T)((kStartF64 / (double)HL) * ((double)nl / (double)kFastKappa))
+                      : (T)(kStartF64 / (double)HL);
     }
     // boundary injection: a group's first lane takes row 0 (M = I = 0,
     // D = C0) instead of its left neighbour's last row -- as a multiply by a
@@ -410,7 +411,8 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
             }
         } else {
             const double a = (double)acc;
-            bd.scores[it.out] = (float)(log10(a) - log10((double)beta) - kLog10StartF64);
+            bd.scores[it.out] = FAST ? (float)(log10(a) - (kLog10StartF64 - log10((double)kFastKappa)))
+                                     : (float)(log10(a) - log10((double)beta) - kLog10StartF64);
             // one scale for the whole alignment: accept sums that stay far
             // above double's floor (score above ~ -572), where no cell that
             // matters can have been flushed; the rest go to the
@@ -872,10 +874,10 @@ static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs,
     for (int i = 0; i < bs.n; ++i)
         if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
             return hipErrorInvalidValue;
-    if (fast && sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_fwd_tile_set<float, NR, true>),
-                           dim3(grid_of(grid, set_blocks_per_cu<float, NR, true>())), dim3(64), 0,
-                           st, p, bs);
+    if (fast) {
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true>())), dim3(64), 0, st,
+                           p, bs);
     } else {
         hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false>),
                            dim3(grid_of(grid, set_blocks_per_cu<T, NR, false>())), dim3(64), 0, st,
@@ -900,20 +902,23 @@ hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinS
     }
 }
 
-hipError_t launch_fwd_tile_set_f64(int NR, const FwdParams& p, const BinSet& bs, int grid,
-                                   hipStream_t st)
+hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                   int grid, hipStream_t st)
 {
     switch (NR) {
-    case 4: return launch_set_nr<double, 4>(false, p, bs, grid, st);
-    case 8: return launch_set_nr<double, 8>(false, p, bs, grid, st);
+    case 4: return launch_set_nr<double, 4>(fast, p, bs, grid, st);
+    case 8: return launch_set_nr<double, 8>(fast, p, bs, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 int tile_set_blocks_per_cu(int NR, bool fast, bool f64)
 {
-    if (f64)
-        return NR == 4 ? set_blocks_per_cu<double, 4, false>() : set_blocks_per_cu<double, 8, false>();
+    if (f64) {
+        if (NR == 4)
+            return fast ? set_blocks_per_cu<double, 4, true>() : set_blocks_per_cu<double, 4, false>();
+        return fast ? set_blocks_per_cu<double, 8, true>() : set_blocks_per_cu<double, 8, false>();
+    }
     switch (NR) {
     case 4: return fast ? set_blocks_per_cu<float, 4, true>() : set_blocks_per_cu<float, 4, false>();
     case 8: return fast ? set_blocks_per_cu<float, 8, true>() : set_blocks_per_cu<float, 8, false>();

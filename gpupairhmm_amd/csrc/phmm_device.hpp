@@ -204,8 +204,8 @@ struct BinSet {
 // grid: 0 = as many blocks as fit on the device at once
 hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                hipStream_t st);
-hipError_t launch_fwd_tile_set_f64(int NR, const FwdParams& p, const BinSet& bs, int grid,
-                                   hipStream_t st);
+hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                   int grid, hipStream_t st);
 hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
                                float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
