@@ -1078,11 +1078,11 @@ extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, 
     int prev = 0;
     for (int i = 0; i < n; ++i) {
         const int tile = t[i].lanes * t[i].rows;
-        if (t[i].lanes < 1 || t[i].lanes > 64 || t[i].rows < 4 || t[i].rows > 32 ||
-            t[i].rows % 4 != 0 || t[i].max_read_len <= prev || tile < t[i].max_read_len + 1)
+        if (t[i].lanes < 1 || t[i].lanes > 64 || !phmm::tile_rows_supported(t[i].rows) ||
+            t[i].max_read_len <= prev || tile < t[i].max_read_len + 1)
             return phmm::fail(PHMM_EINVAL,
-                              "phmm_engine_set_tiles: lanes 1..64, rows 4..32 step 4, max_read_len "
-                              "ascending and <= lanes*rows - 1");
+                              "phmm_engine_set_tiles: lanes 1..64, rows 4, 8, 12, 13, 16, 19, 20, "
+                              "24, 28 or 32, max_read_len ascending and <= lanes*rows - 1");
         prev = t[i].max_read_len;
     }
     std::lock_guard<std::mutex> lk(h->mu);

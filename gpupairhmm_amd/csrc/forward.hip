@@ -144,7 +144,7 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
     constexpr int K = TT::K;
-    constexpr int NQ = NR / K;
+    constexpr int NQ = (NR + K - 1) / K;  // the last vector may hold fewer than K rows
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     int L = 0, roff = 0;
@@ -239,8 +239,12 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int j = K * q + k;
-                    const bool m = (l == code[j]) || (code[j] == 4);
-                    v[k] = m ? lm[j] : lx[j];
+                    if (j < NR) {
+                        const bool m = (l == code[j]) || (code[j] == 4);
+                        v[k] = m ? lm[j] : lx[j];
+                    } else {
+                        v[k] = 0;  // past the lane's rows: never used
+                    }
                 }
                 tab[(l * NQ + q) * 64 + lane] = TT::pack(v);
             }
@@ -266,7 +270,7 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     using TT = TileTraits<T>;
     using V = typename TT::V;
     constexpr int K = TT::K;
-    constexpr int NQ = NR / K;
+    constexpr int NQ = (NR + K - 1) / K;  // the last vector may hold fewer than K rows
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     const bool first = (t == 0);
@@ -357,6 +361,8 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int j = K * qq + k;
+                    if (j >= NR)
+                        break;
                     const T mo = M[j], dov = D[j];
                     T t1;
                     if (FAST) {
@@ -547,8 +553,9 @@ __device__ __forceinline__ int next_task(int32_t* ctr)
 template <typename T, int NR, bool FAST>
 __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
 {
-    static_assert(NR % TileTraits<T>::K == 0 && NR >= 4 && NR <= 32, "rows per lane");
-    __shared__ typename TileTraits<T>::V tab[4 * (NR / TileTraits<T>::K) * 64];
+    static_assert(NR >= 4 && NR <= 32, "rows per lane");
+    constexpr int NQ = (NR + TileTraits<T>::K - 1) / TileTraits<T>::K;
+    __shared__ typename TileTraits<T>::V tab[4 * NQ * 64];
     SetLane sl;
     const int total = set_lanes(bs, bs.st ? 1 : 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
@@ -893,7 +900,9 @@ hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinS
     case 4: return launch_set_nr<float, 4>(fast, p, bs, grid, st);
     case 8: return launch_set_nr<float, 8>(fast, p, bs, grid, st);
     case 12: return launch_set_nr<float, 12>(fast, p, bs, grid, st);
+    case 13: return launch_set_nr<float, 13>(fast, p, bs, grid, st);
     case 16: return launch_set_nr<float, 16>(fast, p, bs, grid, st);
+    case 19: return launch_set_nr<float, 19>(fast, p, bs, grid, st);
     case 20: return launch_set_nr<float, 20>(fast, p, bs, grid, st);
     case 24: return launch_set_nr<float, 24>(fast, p, bs, grid, st);
     case 28: return launch_set_nr<float, 28>(fast, p, bs, grid, st);
@@ -923,7 +932,9 @@ int tile_set_blocks_per_cu(int NR, bool fast, bool f64)
     case 4: return fast ? set_blocks_per_cu<float, 4, true>() : set_blocks_per_cu<float, 4, false>();
     case 8: return fast ? set_blocks_per_cu<float, 8, true>() : set_blocks_per_cu<float, 8, false>();
     case 12: return fast ? set_blocks_per_cu<float, 12, true>() : set_blocks_per_cu<float, 12, false>();
+    case 13: return fast ? set_blocks_per_cu<float, 13, true>() : set_blocks_per_cu<float, 13, false>();
     case 16: return fast ? set_blocks_per_cu<float, 16, true>() : set_blocks_per_cu<float, 16, false>();
+    case 19: return fast ? set_blocks_per_cu<float, 19, true>() : set_blocks_per_cu<float, 19, false>();
     case 20: return fast ? set_blocks_per_cu<float, 20, true>() : set_blocks_per_cu<float, 20, false>();
     case 24: return fast ? set_blocks_per_cu<float, 24, true>() : set_blocks_per_cu<float, 24, false>();
     case 28: return fast ? set_blocks_per_cu<float, 28, true>() : set_blocks_per_cu<float, 28, false>();

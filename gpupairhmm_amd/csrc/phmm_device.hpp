@@ -51,6 +51,13 @@ constexpr int kMaxTiles = 32;
 // Bins: 2 per tile class (FAST, SAFE), the multi-tile bin, and the exact
 // bin (reference CPU semantics, bytes other than A/C/G/T/N present).
 constexpr int kMaxBins = 2 * kMaxTiles + 2;
+// Rows per lane with a compiled tile kernel: multiples of 4, plus 13 and 19
+// (8 lanes of them hold reads of 103 and 151 bases, common sequencer read
+// lengths, with at most one spare row).
+__host__ __device__ inline bool tile_rows_supported(int r)
+{
+    return (r >= 4 && r <= 32 && r % 4 == 0) || r == 13 || r == 19;
+}
 constexpr int kTaskCounters = 64;   // set launches per call (plain mode: one per bin)
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 constexpr int kAlignParts = 64;     // partial counters per bin (atomic spreading)

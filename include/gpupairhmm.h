@@ -140,7 +140,7 @@ typedef struct phmm_stats {
  * lanes x `rows` rows per lane; lanes*rows must be >= max_read_len + 1. */
 typedef struct phmm_tile_config {
     int32_t lanes;         /* lanes per alignment, 1..64 */
-    int32_t rows;          /* rows per lane: 4, 8, 12, 16, 20, 24, 28 or 32 */
+    int32_t rows;          /* rows per lane: 4, 8, 12, 13, 16, 19, 20, 24, 28 or 32 */
     int32_t max_read_len;  /* ascending across the table */
 } phmm_tile_config;
 

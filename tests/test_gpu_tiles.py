@@ -1,5 +1,6 @@
 """Every compiled tile shape against the FP64 oracle: each rows-per-lane
-instantiation (4..32) with several lane counts, both arithmetic forms (the
+instantiation (multiples of 4 from 4 to 32, and 13 and 19, whose last LDS
+vector is partly unused) with several lane counts, both arithmetic forms (the
 rescaled 5-op form and the 6-op form the read pass picks when a read's
 coefficients would leave the rescaled range), and the FP64 tile kernel."""
 import numpy as np
@@ -9,7 +10,7 @@ from conftest import say
 
 pytestmark = pytest.mark.gpu
 
-ROWS = (4, 8, 12, 16, 20, 24, 28, 32)
+ROWS = (4, 8, 12, 13, 16, 19, 20, 24, 28, 32)
 
 
 def _batch(G, max_len, seed):
