@@ -406,12 +406,11 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
 
 __device__ __forceinline__ bool is_tile_bin(const BinParams& bp, int k) { return k < 2 * bp.ntiles; }
 // A tile bin's `reads` reads of a group split into chunks of W = 64/lanes
-// reads: one super-task per full chunk and hap segment.  A short last chunk
-// of n = reads % W reads runs as rep = W/n copies that sweep rep haps at a
-// time, so it takes a run of rep consecutive segments per super-task (at
-// the run's first segment); its sort key is the run's columns per copy.
+// reads: one super-task per full chunk and hap segment.  The n = reads % W
+// reads left over become one-alignment items of the bin's item bin (packed
+// W to a task across records, so no group of a wave idles for them).
 struct ChunkPlan {
-    int W, nfull, rem, rep;
+    int W, nfull, rem;
 };
 __device__ __forceinline__ ChunkPlan chunk_plan(const BinParams& bp, int k, int reads)
 {
@@ -419,20 +418,10 @@ __device__ __forceinline__ ChunkPlan chunk_plan(const BinParams& bp, int k, int 
     c.W = 64 / bp.lanes[k >> 1];
     c.nfull = reads / c.W;
     c.rem = reads - c.nfull * c.W;
-    c.rep = c.rem > 0 ? c.W / c.rem : 1;
     return c;
 }
-__device__ __forceinline__ bool rem_task_at(const ChunkPlan& c, int sg)
-{
-    return c.rem > 0 && sg % c.rep == 0;
-}
-__device__ __forceinline__ int rem_work(const GroupHaps& gh, const ChunkPlan& c, int sg)
-{
-    int w = 0;
-    for (int i = sg; i < min(sg + c.rep, gh.nseg); ++i)
-        w += gh.work[i];
-    return (w + c.rep - 1) / c.rep;
-}
+// The item bin of tile bin k (work-bin ids past the multi-tile and exact bins).
+__device__ __forceinline__ int item_bin_of(const BinParams& bp, int k) { return bp.NB + k; }
 
 // Per group (one wave): the sort-key histogram and the per-bin alignment
 // counts.  A tile bin gets one key (bin, total hap length) per group, counted
@@ -475,11 +464,10 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
                 const ChunkPlan c = chunk_plan(bp, k, gb.cnt[e]);
                 if (c.nfull > 0)
                     atomicAdd(&bp.key_count[key_of(k, gh.work[sg])], c.nfull);
-                if (rem_task_at(c, sg))
-                    atomicAdd(&bp.key_count[key_of(k, rem_work(gh, c, sg))], 1);
             }
         }
-        // per (hap, bin) item run of the one-wave-per-alignment bins
+        // per (hap, bin) item run: the one-wave bins' alignments, and a tile
+        // bin's left-over reads against each of its haps
         for (int x = threadIdx.x; x < nh * n; x += 64) {
             const int h = x / n, e = x - h * n;
             const int k = gb.k[e];
@@ -488,8 +476,12 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             hap_info(bp, b, gh, h0, h, hl, other);
             if (hl <= 0 || k == bp.NB)
                 continue;
-            if (is_tile_bin(bp, k) && !other)
-                continue;  // a super-task covers it
+            if (is_tile_bin(bp, k) && !other) {
+                const ChunkPlan c = chunk_plan(bp, k, gb.cnt[e]);
+                if (c.rem > 0)
+                    atomicAdd(&bp.key_count[key_of(item_bin_of(bp, k), hl)], c.rem);
+                continue;
+            }
             atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl)], gb.cnt[e]);
         }
         __syncthreads();
@@ -504,7 +496,7 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
     __shared__ int wsum[16];
     __shared__ int carry;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int bin = blockIdx.x; bin < bp.NB; bin += gridDim.x) {
+    for (int bin = blockIdx.x; bin < bp.NW; bin += gridDim.x) {
         if (threadIdx.x == 0)
             carry = 0;
         __syncthreads();
@@ -545,7 +537,8 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
 }
 
 // Bin begins: exclusive scans of the bin counts -- in work units
-// (bin_range) and in alignments (align_range) -- one wave, NB <= 66.
+// (bin_range, every work bin) and in alignments (align_range, the bins of
+// reads) -- one wave.
 // Counts come from range[2k+1], or from cnt[k] (then also stored there).
 __device__ __forceinline__ void scan_pairs(int32_t* range, int NB, const int* cnt)
 {
@@ -571,7 +564,7 @@ __device__ __forceinline__ void scan_pairs(int32_t* range, int NB, const int* cn
 __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 {
     __shared__ int tot[kMaxBins];
-    scan_pairs(bp.bin_range, bp.NB, nullptr);
+    scan_pairs(bp.bin_range, bp.NW, nullptr);
     for (int k = threadIdx.x; k < bp.NB; k += 64) {  // each bin's alignment count from its parts
         int v = 0;
 #pragma unroll 16
@@ -584,11 +577,11 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 }
 
 // Per group (one wave): write the work lists.  Tile bins: the group's reads
-// of the bin in chunks of 64/lanes, one super-task per chunk and hap
-// segment (a short last chunk: per run of segments, as chunk_plan says).  The
-// multi-tile and exact bins: one item per alignment, reserved per (hap, bin)
-// run -- one lane per pair, haps in chunks that fit the LDS table -- and
-// written with lanes over the reads of a run.  Alignments with an empty read
+// of the bin in full chunks of 64/lanes, one super-task per chunk and hap
+// segment; the reads left over go to the bin's item bin.  Item bins, the
+// multi-tile and the exact bins: one item per alignment, reserved per (hap,
+// bin) run -- one lane per pair, haps in chunks that fit the LDS table --
+// and written with lanes over the reads of a run.  Alignments with an empty read
 // or hap get -inf directly (align_host returns log10(0) - log10(c) for them).
 __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
 {
@@ -598,6 +591,7 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
     __shared__ GroupBins gb;
     __shared__ GroupHaps gh;
     __shared__ int run_x[kRuns], run_base[kRuns];
+    __shared__ int w_off[64], w_st[64], w_h[64], w_base[64];
     const int lane = threadIdx.x;
     const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
@@ -633,14 +627,6 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                     bp.st[base + j] = t;
                 }
             }
-            if (rem_task_at(cp, sg)) {
-                const int rbase = bp.bin_range[2 * k] +
-                                  atomicAdd(&bp.key_cursor[key_of(k, rem_work(gh, cp, sg))], 1);
-                t.perm = r0 + gb.st[e] + cp.nfull * cp.W;
-                t.nc = cp.rem;
-                t.he = gh.he[min(sg + cp.rep, gh.nseg) - 1];
-                bp.st[rbase] = t;
-            }
         }
         // item runs and -inf: lanes reserve the runs of (hap, bin entry)
         // pairs -- tile-bin pairs are covered by super-tasks -- and list the
@@ -662,7 +648,13 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                     base = -1;  // -inf
                     if (hl > 0 && k != bp.NB) {
                         if (is_tile_bin(bp, k) && !other) {
-                            base = kSkip;
+                            // super-tasks cover the full chunks; left-over
+                            // reads become items of the bin's item bin
+                            const ChunkPlan cp = chunk_plan(bp, k, gb.cnt[e]);
+                            const int kb = item_bin_of(bp, k);
+                            base = cp.rem == 0 ? kSkip
+                                               : bp.bin_range[2 * kb] +
+                                                     atomicAdd(&bp.key_cursor[key_of(kb, hl)], cp.rem);
                         } else {
                             const int kb = is_tile_bin(bp, k) ? eb : k;
                             base = bp.bin_range[2 * kb] +
@@ -680,24 +672,68 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                 nrun += __popcll(m);
             }
             __syncthreads();
-            for (int q = 0; q < nrun; ++q) {
-                const int x = run_x[q], base = run_base[q];
-                const int h = hc + x / n, e = x % n;
-                const int cnt = gb.cnt[e], st = gb.st[e];
-                for (int j = lane; j < cnt; j += 64) {
-                    const int rd = bp.read_perm[r0 + st + j];
-                    const int64_t out = A + (int64_t)(rd - r0) * nh + h;
-                    if (base < 0) {
+            // write the listed runs 64 at a time, lanes over the runs' (run,
+            // read) pairs: many runs are a read or two (left-over reads), and
+            // every pair costs a dependent load of read_perm
+            for (int q0 = 0; q0 < nrun; q0 += 64) {  // same trip count on every lane
+                const int q = q0 + lane;
+                int cnt = 0, st = 0, h = 0, base = 0;
+                if (q < nrun) {
+                    const int x = run_x[q];
+                    const int e = x % n;
+                    h = hc + x / n;
+                    base = run_base[q];
+                    cnt = gb.cnt[e];
+                    st = gb.st[e];
+                    if (base >= 0 && is_tile_bin(bp, gb.k[e])) {
+                        int hl;
+                        bool other;
+                        hap_info(bp, b, gh, h0, h, hl, other);
+                        if (!other) {  // an item-bin run: the left-over reads only
+                            const ChunkPlan cp = chunk_plan(bp, gb.k[e], cnt);
+                            st += cp.nfull * cp.W;
+                            cnt = cp.rem;
+                        }
+                    }
+                }
+                int incl = cnt;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(incl, o, 64);
+                    if (lane >= o)
+                        incl += y;
+                }
+                const int total = __shfl(incl, 63, 64);
+                w_off[lane] = incl - cnt;
+                w_st[lane] = st;
+                w_h[lane] = h;
+                w_base[lane] = base;
+                __syncthreads();
+                const int last = min(64, nrun - q0) - 1;
+                for (int f = lane; f < total; f += 64) {
+                    int lo = 0, hi = last;  // the last run whose offset is <= f
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (w_off[mid] <= f)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const int j = f - w_off[lo];
+                    const int rd = bp.read_perm[r0 + w_st[lo] + j];
+                    const int64_t out = A + (int64_t)(rd - r0) * nh + w_h[lo];
+                    if (w_base[lo] < 0) {
                         b.scores[out] = -INFINITY;  // empty read or empty hap
                     } else {
                         WorkItem it;
                         it.read = rd;
-                        it.hap = h0 + h;
+                        it.hap = h0 + w_h[lo];
                         it.out = (int32_t)out;
                         it.pad = 0;
-                        bp.items[base + j] = it;
+                        bp.items[w_base[lo] + j] = it;
                     }
                 }
+                __syncthreads();
             }
             __syncthreads();
         }
@@ -733,8 +769,8 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         int64_t b = (n + per_block - 1) / per_block;
         return (int)(b < cap ? (b < 1 ? 1 : b) : cap);
     };
-    hipLaunchKernelGGL(k_clear, dim3(grid((int64_t)bp.NB * kHapBuckets, tpb, 512)), dim3(tpb), 0,
-                       st, bp.key_count, (int64_t)bp.NB * kHapBuckets);
+    hipLaunchKernelGGL(k_clear, dim3(grid((int64_t)bp.NW * kHapBuckets, tpb, 512)), dim3(tpb), 0,
+                       st, bp.key_count, (int64_t)bp.NW * kHapBuckets);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.hmax, (int64_t)1);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
@@ -755,7 +791,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
         hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
     }
-    hipLaunchKernelGGL(k_scan_keys, dim3(bp.NB), dim3(1024), 0, st, bp);
+    hipLaunchKernelGGL(k_scan_keys, dim3(bp.NW), dim3(1024), 0, st, bp);
     hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(64), 0, st, bp);
     if (fixed_grids || hd.G > 0)
         hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);

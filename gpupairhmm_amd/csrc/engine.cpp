@@ -199,6 +199,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
                          "(byte counts multiples of 4)");
     const int ntiles = (int)tiles.size();
     const int NB = 2 * ntiles + 2;  // + multi-tile bin + exact bin
+    const int NW = NB + 2 * ntiles; // + an item bin per tile bin
     const int G = v.num_groups;
     const int64_t nA = v.num_alignments;
     int rc;
@@ -209,9 +210,9 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(gb_start, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
         (rc = ensure(read_perm, sizeof(int32_t) * (size_t)std::max(v.num_reads, 1), "read perm")) ||
         (rc = ensure(align_off, sizeof(int64_t) * (size_t)(G + 1), "align offsets")) ||
-        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
-        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NB * kHapBuckets, "keys")) ||
-        (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NB, "bin ranges")) ||
+        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NW * kHapBuckets, "keys")) ||
+        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NW * kHapBuckets, "keys")) ||
+        (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NW, "bin ranges")) ||
         (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
         (rc = ensure(align_range, sizeof(int32_t) * (2 + kAlignParts) * (size_t)NB,
@@ -292,6 +293,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     BinParams bp{};
     bp.bd = static_cast<const BatchDesc*>(desc.p);
     bp.NB = NB;
+    bp.NW = NW;
     bp.rows = static_cast<uint32_t*>(rows.p);
     bp.hapc = static_cast<uint8_t*>(hapc.p);
     bp.meta = static_cast<ReadMeta*>(meta.p);
@@ -357,10 +359,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
             // one host round trip per batch for the per-bin sizes (the
             // reference syncs per chunk for its per-partition counts,
             // gpuworker.cuh:852-854)
-            if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NB,
+            if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NW,
                                            hipMemcpyDeviceToHost, st),
                             "bin range readback")) ||
-                (rc = check(hipMemcpyAsync(h_bin_range + 2 * kMaxBins, bp.align_range,
+                (rc = check(hipMemcpyAsync(h_bin_range + 2 * kMaxWorkBins, bp.align_range,
                                            sizeof(int32_t) * 2 * NB, hipMemcpyDeviceToHost, st),
                             "bin range readback")) ||
                 (rc = check(hipStreamSynchronize(st), "bin range sync")))
@@ -430,11 +432,14 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
     const int eb = mb + 1;      // exact bin
     float* spill32 = static_cast<float*>(spill.p);
     const int64_t n32 = (int64_t)(spill.cap / sizeof(float));
-    const int32_t* h_align = h_bin_range + 2 * kMaxBins;
+    const int32_t* h_align = h_bin_range + 2 * kMaxWorkBins;
+    const int NB = bp.NB;  // item bin of tile bin b: NB + b
     int rc;
     int ctr = 0;
     auto tile_set = [&](BinSet& bs) {
         bs.st = bp.st;
+        bs.items = bp.items;
+        bs.item_bin0 = NB;
         bs.range = bp.bin_range;
         bs.counts = bp.bin_range + 1;
         bs.count_stride = 2;
@@ -442,16 +447,18 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
     };
     if (plain) {
         for (int b = 2 * ntiles - 1; b >= 0; --b) {
-            const int tasks = h_bin_range[2 * b + 1];
+            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
+            const int W = 64 / tc.lanes;
+            const int tasks = h_bin_range[2 * b + 1] + (h_bin_range[2 * (NB + b) + 1] + W - 1) / W;
             if (tasks <= 0)
                 continue;
-            const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
             if (b % 2 == 0)
                 stats.fast_alignments += h_align[2 * b + 1];
             BinSet bs;
-            bs.n = 1;
+            bs.n = 2;
             bs.bin[0] = b;
-            bs.lanes[0] = tc.lanes;
+            bs.bin[1] = NB + b;
+            bs.lanes[0] = bs.lanes[1] = tc.lanes;
             tile_set(bs);
             if (ctr >= kTaskCounters - 4)
                 return set_error(PHMM_ERANGE, "too many set launches");
@@ -472,13 +479,16 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                 seen = tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast;
             if (seen)
                 continue;
+            // the template's super-task bins, larger tiles first, then their
+            // item bins (one sweep per task: they fill the launch's tail)
             BinSet bs;
-            for (int b = b0; b >= 0; --b)
-                if (tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast) {
-                    bs.bin[bs.n] = b;
-                    bs.lanes[bs.n] = tiles[(size_t)(b / 2)].lanes;
-                    ++bs.n;
-                }
+            for (int pass = 0; pass < 2; ++pass)
+                for (int b = b0; b >= 0; --b)
+                    if (tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast) {
+                        bs.bin[bs.n] = pass == 0 ? b : NB + b;
+                        bs.lanes[bs.n] = tiles[(size_t)(b / 2)].lanes;
+                        ++bs.n;
+                    }
             tile_set(bs);
             if (ctr >= kTaskCounters - 4)
                 return set_error(PHMM_ERANGE, "too many set launches");
@@ -518,7 +528,7 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     const int ntiles = (int)tiles.size();
     double* spill64 = static_cast<double*>(spill.p);
     const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
-    const int32_t* h_align = h_bin_range + 2 * kMaxBins;
+    const int32_t* h_align = h_bin_range + 2 * kMaxWorkBins;
     int rc;
     // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
     // the bin's form: FAST-eligible reads recompute in the FAST form)
@@ -1009,7 +1019,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
         (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipHostMalloc(&e.h_bin_range, 4 * sizeof(int32_t) * phmm::kMaxBins,
+        (rc = e.check(hipHostMalloc(&e.h_bin_range,
+                                    2 * sizeof(int32_t) * (phmm::kMaxWorkBins + phmm::kMaxBins),
                                     hipHostMallocDefault),
                       "pinned")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")))

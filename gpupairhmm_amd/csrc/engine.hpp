@@ -40,7 +40,8 @@ struct Engine {
     DevBuf rows, hapc, meta, gb_count, gb_start, read_perm, align_off, key_count, key_cursor,
         bin_range, align_range, smallints, items, supertasks, rescue_items, final_items, spill,
         hapflag;
-    int32_t* h_bin_range = nullptr;   // pinned, 2*NB work ranges then 2*NB alignment ranges
+    int32_t* h_bin_range = nullptr;   // pinned: 2*kMaxWorkBins work ranges, then 2*kMaxBins
+                                      // alignment ranges
     int32_t* h_small = nullptr;       // pinned, small readbacks
     // event pairs around the forward kernels of timed calls
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_pending;

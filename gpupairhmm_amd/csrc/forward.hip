@@ -460,11 +460,9 @@ __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename Til
 }
 
 // A super-task: the groups set up their reads once and sweep the segment's
-// haps in turn.  A chunk of nc < 64/G reads (a record's last) spreads over
-// rep = (64/G)/nc copies of itself: copy q sweeps haps hs+q, hs+q+rep, ...,
-// so the groups stay busy.  Empty haps were scored -inf by the scatter;
-// under reference CPU semantics, haps holding a byte other than A/C/G/T/N
-// belong to the exact bin.
+// haps in turn (all groups share each hap, so every sweep is uniform across
+// the wave).  Empty haps were scored -inf by the scatter; haps holding a
+// byte other than A/C/G/T belong to the exact bin.
 template <typename T, int NR, bool FAST>
 __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const SuperTask& st, int G, int rescue_base, int bin)
@@ -474,33 +472,29 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
     const int W = 64 / G;
     const int grp = lane / G;
     const int t = lane - grp * G;
-    const int rep = W / st.nc;  // 1 <= nc <= W
-    const int q = grp / st.nc;
-    const bool active = grp < W && q < rep;
-    const int rd = active ? p.read_perm[st.perm + (grp - q * st.nc)] : 0;
+    const bool active = grp < W && grp < st.nc;
+    const int rd = active ? p.read_perm[st.perm + grp] : 0;
     TileRead<T, NR> R;
     tile_setup<T, NR, FAST>(p, tab, active, rd, t, R);
-    for (int hb = st.hs; hb < st.he; hb += rep) {  // same trip count on every lane
-        const int h = hb + q;
-        const int hap = st.h0 + min(h, st.he - 1);
+    for (int h = st.hs; h < st.he; ++h) {
+        const int hap = st.h0 + h;
         const int HL = bd.hap_len[hap];
-        const bool go = active && h < st.he && HL > 0 && p.hapflag[hap] == 0;
-        if (__ballot(go) == 0ull)
+        if (HL <= 0 || p.hapflag[hap] != 0)
             continue;
         WorkItem it;
         it.read = rd;
         it.hap = hap;
         it.out = st.aoff + (rd - st.r0) * st.nh + h;
         it.pad = 0;
-        tile_sweep<T, NR, FAST>(p, tab, R, go, G, t, go ? HL : 0, go ? bd.hap_off[hap] : 0, it,
-                                rescue_base, bin);
+        tile_sweep<T, NR, FAST>(p, tab, R, active, G, t, HL, bd.hap_off[hap], it, rescue_base, bin);
     }
 }
 
 // Lane i < bs.n of a wave describes bin i of a BinSet: its id, lanes per
-// alignment, begin, count and number of tasks (`per_task` items per task,
-// 0: 64 / lanes alignments per task); incl is the inclusive prefix of tasks
-// over the lanes.  Returns the total number of tasks.
+// alignment, begin, count and number of tasks (`per_task` units per task;
+// 0: one super-task per task in super-task bins, 64 / lanes items per task
+// in the others); incl is the inclusive prefix of tasks over the lanes.
+// Returns the total number of tasks.
 struct SetLane {
     int bin, G, begin, count, ntask, incl;
 };
@@ -517,7 +511,9 @@ __device__ __forceinline__ int set_lanes(const BinSet& bs, int per_task, SetLane
         s.G = bs.lanes[lane] > 0 ? bs.lanes[lane] : 1;
         s.count = bs.counts[(int64_t)bs.count_stride * s.bin];
         s.begin = bs.range ? bs.range[2 * s.bin] : 0;
-        const int per = per_task > 0 ? per_task : 64 / s.G;
+        const int per = per_task > 0                            ? per_task
+                        : (bs.st && s.bin < bs.item_bin0) ? 1
+                                                          : 64 / s.G;
         s.ntask = (s.count + per - 1) / per;
     }
     int x = s.ntask;
@@ -557,7 +553,7 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
     constexpr int NQ = (NR + TileTraits<T>::K - 1) / TileTraits<T>::K;
     __shared__ typename TileTraits<T>::V tab[4 * NQ * 64];
     SetLane sl;
-    const int total = set_lanes(bs, bs.st ? 1 : 0, sl);
+    const int total = set_lanes(bs, 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
         return;
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
@@ -567,12 +563,14 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
         const int begin = __shfl(sl.begin, k, 64);
         const int count = __shfl(sl.count, k, 64);
         const int t0 = __shfl(sl.incl - sl.ntask, k, 64);
-        const int rescue_base = bs.seg ? bs.seg[2 * bin] : 0;
-        if (bs.st)
-            tile_super_task<T, NR, FAST>(p, tab, bs.st[begin + (task - t0)], G, rescue_base, bin);
+        // an item bin's results below the threshold join its tile bin's list
+        const int rbin = bin >= bs.item_bin0 ? bin - bs.item_bin0 : bin;
+        const int rescue_base = bs.seg ? bs.seg[2 * rbin] : 0;
+        if (bs.st && bin < bs.item_bin0)
+            tile_super_task<T, NR, FAST>(p, tab, bs.st[begin + (task - t0)], G, rescue_base, rbin);
         else
             tile_items_task<T, NR, FAST>(p, tab, bs.items, begin, count, G, task - t0, rescue_base,
-                                         bin);
+                                         rbin);
     }
 }
 

@@ -51,6 +51,9 @@ constexpr int kMaxTiles = 32;
 // Bins: 2 per tile class (FAST, SAFE), the multi-tile bin, and the exact
 // bin (reference CPU semantics, bytes other than A/C/G/T/N present).
 constexpr int kMaxBins = 2 * kMaxTiles + 2;
+// Work bins: the bins above, then one item bin per tile bin (the alignments
+// of a record's reads left over after full chunks of 64/lanes reads).
+constexpr int kMaxWorkBins = kMaxBins + 2 * kMaxTiles;
 // Rows per lane with a compiled tile kernel: multiples of 4, plus 13 and 19
 // (8 lanes of them hold reads of 103 and 151 bases, common sequencer read
 // lengths, with at most one spare row).
@@ -152,7 +155,8 @@ struct FwdParams {
 // Everything the binning kernels read and write (binning.hip).
 struct BinParams {
     const BatchDesc* bd;
-    int32_t NB;
+    int32_t NB;                // bins of reads: 2*ntiles + 2
+    int32_t NW;                // work bins: NB + 2*ntiles (item bin of tile bin k: NB + k)
     // scratch
     uint32_t* rows;
     uint8_t* hapc;
@@ -161,9 +165,9 @@ struct BinParams {
     int32_t* gb_start;         // [G*(NB+1)]
     int32_t* read_perm;        // [nR]
     int64_t* align_off;        // [G+1]
-    int32_t* key_count;        // [NB*kHapBuckets]
-    int32_t* key_cursor;       // [NB*kHapBuckets]
-    int32_t* bin_range;        // [2*NB]: begin, count (tile bins: super-tasks; others: items)
+    int32_t* key_count;        // [NW*kHapBuckets]
+    int32_t* key_cursor;       // [NW*kHapBuckets]
+    int32_t* bin_range;        // [2*NW]: begin, count (tile bins: super-tasks; others: items)
     int32_t* hmax;             // [1]
     int32_t* rescue_counts;    // [NB]
     int32_t* final_count;      // [1]
@@ -172,7 +176,7 @@ struct BinParams {
     SuperTask* st;             // tile bins' work, in bin_range units
     int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
     int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin
-    uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T/N (CPU semantics)
+    uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T
     WorkItem* items;
     const float* ph2pr;
     // tiles
@@ -191,13 +195,16 @@ hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStr
 // Several bins in one persistent launch.  Bin bin[i] holds count_i =
 // counts[count_stride*bin[i]] units starting at begin_i = range ?
 // range[2*bin[i]] : 0, both read on the device.  A unit is a SuperTask
-// (st != null: tile kernels, one task each) or a WorkItem of `items` (tile
-// kernels: 64/lanes per task; wave kernels: one each).  The bins' tasks form
-// one sequence in the given order; tile-kernel waves take the next task from
-// *next_task (zeroed before the launch).  seg[bin] is where a tile bin's
-// FP32 rescue list starts in FwdParams::rescue_items.
+// (st != null and bin < item_bin0: tile kernels, one task each) or a
+// WorkItem of `items` (tile kernels: 64/lanes per task; wave kernels: one
+// each).  The bins' tasks form one sequence in the given order; tile-kernel
+// waves take the next task from *next_task (zeroed before the launch).
+// Rescue lists: results of bin b (an item bin: of its tile bin b -
+// item_bin0) below the threshold go to tile bin's list, which starts at
+// seg[2*bin] in FwdParams::rescue_items.
 struct BinSet {
     int32_t n = 0;
+    int32_t item_bin0 = 1 << 30;   // first item bin id (tile kernels with st)
     int32_t bin[kMaxBins] = {};
     int32_t lanes[kMaxBins] = {};  // tile kernels: lanes per alignment
     const WorkItem* items = nullptr;
