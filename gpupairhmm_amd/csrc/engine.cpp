@@ -460,7 +460,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             bs.bin[1] = NB + b;
             bs.lanes[0] = bs.lanes[1] = tc.lanes;
             tile_set(bs);
-            if (ctr >= kTaskCounters - 4)
+            if (ctr >= kTaskCounters - kF64Sets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             const int grid =
@@ -490,7 +490,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                         ++bs.n;
                     }
             tile_set(bs);
-            if (ctr >= kTaskCounters - 4)
+            if (ctr >= kTaskCounters - kF64Sets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st), "forward kernel")))
@@ -531,31 +531,62 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     const int32_t* h_align = h_bin_range + 2 * kMaxWorkBins;
     int rc;
     // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
-    // the bin's form: FAST-eligible reads recompute in the FAST form)
-    BinSet set[2][2], wave;  // [NR 4, 8][SAFE, FAST]
-    for (BinSet* x : {&set[0][0], &set[0][1], &set[1][0], &set[1][1], &wave}) {
-        x->items = fp.rescue_items;
-        x->range = bp.align_range;
-        x->counts = fp.rescue_counts;
-        x->count_stride = 1;
-    }
+    // the bin's form: FAST-eligible reads recompute in the FAST form).  Each
+    // bin takes the rows per lane that cost least per alignment: lanes per
+    // wave it occupies (64 / alignments per wave) times the per-step work of
+    // a lane (5 or 6 FP64 ops per row plus about 12 for the exchange).  The
+    // 10-row FAST tile (248 VGPRs) still runs two waves per SIMD; the 10-row
+    // SAFE one would not, so SAFE stops at 8.
+    static constexpr int kNr64[] = {4, 6, 8, 10};
+    BinSet set[4][2], wave;  // [rows index][SAFE, FAST]
+    for (auto& row : set)
+        for (BinSet& x : row) {
+            x.items = fp.rescue_items;
+            x.range = bp.align_range;
+            x.counts = fp.rescue_counts;
+            x.count_stride = 1;
+        }
+    wave.items = fp.rescue_items;
+    wave.range = bp.align_range;
+    wave.counts = fp.rescue_counts;
+    wave.count_stride = 1;
     for (int b = 2 * ntiles - 1; b >= 0; --b) {
         if (plain && h_align[2 * b + 1] <= 0)
             continue;
         const int tile = tiles[(size_t)(b / 2)].max_read_len + 1;  // rows the FP64 tile must hold
-        const int nr64 = tile <= 64 ? 4 : 8;
-        BinSet& x = tile > 512 ? wave : set[nr64 == 8][b % 2 == 0];
+        const bool fast = b % 2 == 0;
+        if (tile > 512) {
+            wave.bin[wave.n] = b;
+            wave.lanes[wave.n] = 64;
+            ++wave.n;
+            continue;
+        }
+        int best = -1;
+        double best_cost = 0;
+        for (int i = 0; i < 4; ++i) {
+            const int nr = kNr64[i];
+            const int G = (tile + nr - 1) / nr;
+            if (G > 64 || (!fast && nr > 8))
+                continue;
+            const double cost = (64.0 / (64 / G)) * ((fast ? 5 : 6) * nr + 12);
+            if (best < 0 || cost < best_cost) {
+                best = i;
+                best_cost = cost;
+            }
+        }
+        BinSet& x = set[best][fast];
         x.bin[x.n] = b;
-        x.lanes[x.n] = tile > 512 ? 64 : (tile + nr64 - 1) / nr64;
+        x.lanes[x.n] = (tile + kNr64[best] - 1) / kNr64[best];
         ++x.n;
     }
-    for (int i = 0; i < 2; ++i)
+    int nset = 0;
+    for (int i = 3; i >= 0; --i)  // more rows per lane first
         for (int f = 0; f < 2; ++f) {
-            BinSet& x = set[1 - i][f];  // 8 rows per lane first
+            BinSet& x = set[i][f];
             if (!x.n)
                 continue;
-            x.next_task = bp.task_ctr + (kTaskCounters - 4 + 2 * i + f);
-            if ((rc = check(launch_fwd_tile_set_f64(i == 0 ? 8 : 4, f == 1, fp, x, 0, st),
+            x.next_task = bp.task_ctr + (kTaskCounters - kF64Sets + nset++);
+            if ((rc = check(launch_fwd_tile_set_f64(kNr64[i], f == 1, fp, x, 0, st),
                             "FP64 tile kernel")))
                 return rc;
         }

@@ -914,7 +914,9 @@ hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const 
 {
     switch (NR) {
     case 4: return launch_set_nr<double, 4>(fast, p, bs, grid, st);
+    case 6: return launch_set_nr<double, 6>(fast, p, bs, grid, st);
     case 8: return launch_set_nr<double, 8>(fast, p, bs, grid, st);
+    case 10: return launch_set_nr<double, 10>(fast, p, bs, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -922,9 +924,13 @@ hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const 
 int tile_set_blocks_per_cu(int NR, bool fast, bool f64)
 {
     if (f64) {
-        if (NR == 4)
-            return fast ? set_blocks_per_cu<double, 4, true>() : set_blocks_per_cu<double, 4, false>();
-        return fast ? set_blocks_per_cu<double, 8, true>() : set_blocks_per_cu<double, 8, false>();
+        switch (NR) {
+        case 4: return fast ? set_blocks_per_cu<double, 4, true>() : set_blocks_per_cu<double, 4, false>();
+        case 6: return fast ? set_blocks_per_cu<double, 6, true>() : set_blocks_per_cu<double, 6, false>();
+        case 8: return fast ? set_blocks_per_cu<double, 8, true>() : set_blocks_per_cu<double, 8, false>();
+        case 10: return fast ? set_blocks_per_cu<double, 10, true>() : set_blocks_per_cu<double, 10, false>();
+        default: return 0;
+        }
     }
     switch (NR) {
     case 4: return fast ? set_blocks_per_cu<float, 4, true>() : set_blocks_per_cu<float, 4, false>();

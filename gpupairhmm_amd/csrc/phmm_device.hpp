@@ -61,7 +61,8 @@ __host__ __device__ inline bool tile_rows_supported(int r)
 {
     return (r >= 4 && r <= 32 && r % 4 == 0) || r == 13 || r == 19;
 }
-constexpr int kTaskCounters = 64;   // set launches per call (plain mode: one per bin)
+constexpr int kTaskCounters = 96;   // set launches per call (plain mode: one per bin)
+constexpr int kF64Sets = 8;         // of them, the last ones: FP64 tile sets
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 constexpr int kAlignParts = 64;     // partial counters per bin (atomic spreading)
 
