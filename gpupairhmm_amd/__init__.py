@@ -79,7 +79,7 @@ class Options(ctypes.Structure):
         ("semantics", ctypes.c_int32), ("fp64_rescue", ctypes.c_int32),
         ("rescue_min_sum", ctypes.c_float), ("verbose", ctypes.c_int32),
         ("time_kernels", ctypes.c_int32), ("graphs", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("small_batch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -491,10 +491,12 @@ class Engine:
     def __init__(self, device: int = 0, mode: int = MODE_BATCHED, semantics: int = SEM_GPU,
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
                  rescue_min_sum: float = FLT_MIN, time_kernels: bool = False,
-                 graphs: bool = True):
+                 graphs: bool = True, small_batch: int | None = None):
         lib = load_library()
         o = Options()
         lib.phmm_options_default(ctypes.byref(o))
+        if small_batch is not None:
+            o.small_batch = int(small_batch)
         o.transfer_batchsize = int(transfer_batchsize)
         o.mode = int(mode)
         o.semantics = int(semantics)

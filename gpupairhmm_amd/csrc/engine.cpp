@@ -197,6 +197,12 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         return set_error(PHMM_EINVAL,
                          "score_device: sequence arrays must be 4-byte aligned with padded lengths "
                          "(byte counts multiples of 4)");
+    // batched calls of small batches use the one-template table (plain mode
+    // keeps the per-class launches of the tuned table)
+    act = (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 && v.num_alignments <= opt.small_batch)
+              ? &tiles_small
+              : &tiles;
+    const std::vector<phmm_tile_config>& tiles = *act;
     const int ntiles = (int)tiles.size();
     const int NB = 2 * ntiles + 2;  // + multi-tile bin + exact bin
     const int NW = NB + 2 * ntiles; // + an item bin per tile bin
@@ -379,9 +385,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     } else {
         // batched: replay captured graphs -- one for the whole call, or one
         // per stage when the forward kernels are timed
+        Graphs& graphs = this->graphs[act == &tiles_small ? 1 : 0];
         if (graphs.gen != generation) {
             drop_graphs();
-            graphs.gen = generation;
+            this->graphs[0].gen = this->graphs[1].gen = generation;
         }
         auto launch = [&](int stages, hipGraphExec_t* ex) -> int {
             int r2;
@@ -427,6 +434,7 @@ int Engine::enqueue_prep(const BinParams& bp, const BatchDesc& hd, bool fixed, h
 // (the reference's per-partition launches).
 int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
 {
+    const std::vector<phmm_tile_config>& tiles = *act;
     const int ntiles = (int)tiles.size();
     const int mb = 2 * ntiles;  // multi-tile bin
     const int eb = mb + 1;      // exact bin
@@ -525,6 +533,7 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
 // finished.
 int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
 {
+    const std::vector<phmm_tile_config>& tiles = *act;
     const int ntiles = (int)tiles.size();
     double* spill64 = static_cast<double*>(spill.p);
     const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
@@ -647,9 +656,22 @@ int Engine::capture(int stages, const BinParams& bp, const FwdParams& fp, const 
     return check(e2, "graph instantiate");
 }
 
+void Engine::derive_small_tiles()
+{
+    tiles_small = tiles;
+    for (auto& t : tiles_small) {
+        const int lanes = (t.max_read_len + 16) / 16;  // ceil((max_read_len + 1) / 16)
+        if (lanes <= 64) {
+            t.lanes = lanes;
+            t.rows = 16;
+        }
+    }
+}
+
 void Engine::drop_graphs()
 {
-    hipGraphExec_t* all[] = {&graphs.all, &graphs.prep, &graphs.fwd, &graphs.rescue};
+    hipGraphExec_t* all[] = {&graphs[0].all, &graphs[0].prep, &graphs[0].fwd, &graphs[0].rescue,
+                             &graphs[1].all, &graphs[1].prep, &graphs[1].fwd, &graphs[1].rescue};
     bool any = false;
     for (auto* x : all)
         any = any || *x;
@@ -1002,6 +1024,7 @@ extern "C" void phmm_options_default(phmm_options* o)
     o->rescue_min_sum = FLT_MIN;  // where the reference's FP32 sum underflows
     o->verbose = 0;
     o->graphs = 1;
+    o->small_batch = 1 << 18;
 }
 
 extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engine** out)
@@ -1032,6 +1055,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
     e.device = device;
     e.opt = o;
     e.tiles.assign(phmm::kDefaultTiles, phmm::kDefaultTiles + phmm::kNumDefaultTiles);
+    e.derive_small_tiles();
     int rc;
     auto bail = [&](int code) {
         const std::string m = e.err.empty() ? phmm::last_error() : e.err;
@@ -1129,6 +1153,7 @@ extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, 
     }
     std::lock_guard<std::mutex> lk(h->mu);
     h->e.tiles.assign(t, t + n);
+    h->e.derive_small_tiles();
     ++h->e.generation;  // captured graphs hold the old table
     return PHMM_OK;
 }

@@ -31,6 +31,11 @@ struct Engine {
     hipStream_t stream = nullptr;     // compute
     hipStream_t copy_stream = nullptr;
     std::vector<phmm_tile_config> tiles;
+    // The one-template table of small batches (options.small_batch): the
+    // classes of `tiles`, 16 rows per lane each (where 64 lanes hold them).
+    std::vector<phmm_tile_config> tiles_small;
+    const std::vector<phmm_tile_config>* act = &tiles;  // the current call's table
+    void derive_small_tiles();
     float* d_ph2pr = nullptr;
     int num_cus = 256;
     std::string err;
@@ -68,7 +73,7 @@ struct Engine {
     struct Graphs {
         uint64_t gen = ~0ull;
         hipGraphExec_t all = nullptr, prep = nullptr, fwd = nullptr, rescue = nullptr;
-    } graphs;
+    } graphs[2];  // [0]: tuned table, [1]: small-batch table
     hipStream_t cap_stream = nullptr;
     int64_t graph_captures = 0, graph_launches = 0;
 

@@ -78,7 +78,14 @@ typedef struct phmm_options {
     int32_t graphs;            /* PHMM_MODE_BATCHED: 1 (default) replays the
                                   call's kernel sequence as one captured HIP
                                   graph; 0 launches the kernels one by one */
-    int32_t reserved[5];
+    int32_t small_batch;       /* PHMM_MODE_BATCHED: batches of at most this
+                                  many alignments use a one-template tile
+                                  table (16 rows per lane for every class),
+                                  so their latency is one launch per stage
+                                  rather than one per tile template; results
+                                  are bit-identical.  0: always the tuned
+                                  table.  Default 1 << 18. */
+    int32_t reserved[4];
 } phmm_options;
 
 /* ---- batch view (BatchOfSequenceGroups, common.cuh:109-201) --------- *

@@ -64,6 +64,18 @@ def test_plain_and_batched_identical(engine, plain_engine):
     assert np.array_equal(a.view(np.uint32), p.view(np.uint32))
 
 
+def test_small_batch_table_identical(lib, engine):
+    """Small batched calls run on the one-template table (16 rows per lane);
+    the tuned table must give the same bits."""
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(2, seed=78, num_records=6)
+    say(f"case: small-batch table vs tuned table, {b.num_alignments} alignments")
+    a = engine.score_batch(b)
+    with G.Engine(small_batch=0) as e2:
+        t = e2.score_batch(b)
+    assert np.array_equal(a.view(np.uint32), t.view(np.uint32))
+
+
 @pytest.mark.parametrize("tb", [1, 7, 1000, 0])
 def test_transfer_chunking_does_not_change_scores(lib, engine, tb):
     import gpupairhmm_amd as G
