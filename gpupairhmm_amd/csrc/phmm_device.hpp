@@ -49,7 +49,8 @@ constexpr double kLog10StartF64 = 301.02999566398120;  // 1000 * log10(2)
 
 constexpr int kMaxTiles = 32;
 // Bins: 2 per tile class (FAST, SAFE), the multi-tile bin, and the exact
-// bin (reference CPU semantics, bytes other than A/C/G/T/N present).
+// bin (haps with a byte other than A/C/G/T; under reference CPU semantics
+// also reads with a byte other than A/C/G/T/N).
 constexpr int kMaxBins = 2 * kMaxTiles + 2;
 // Work bins: the bins above, then one item bin per tile bin (the alignments
 // of a record's reads left over after full chunks of 64/lanes reads).
@@ -141,7 +142,7 @@ struct FwdParams {
     const uint8_t* hapc;         // hap base codes, indexed like hap bytes
     const ReadMeta* meta;
     const int32_t* read_perm;    // reads of each record ordered by bin
-    const uint8_t* hapflag;      // CPU semantics: hap holds a byte other than A/C/G/T/N
+    const uint8_t* hapflag;      // 1: the hap holds a byte other than A/C/G/T (exact bin)
     const float* ph2pr;          // 128 entries
     WorkItem* rescue_items;      // per tile bin: alignments to recompute in FP64
     int32_t* rescue_counts;      // per bin
