@@ -99,6 +99,10 @@ __global__ void k_hap_max(BinParams bp)
 __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
+    __shared__ float ph2pr[128];
+    for (int q = threadIdx.x; q < 128; q += blockDim.x)
+        ph2pr[q] = bp.ph2pr[q];
+    __syncthreads();
     const int sl = threadIdx.x & 15;
     const int nslots = gridDim.x * (blockDim.x >> 4);
     // every 16-lane slot runs the same number of iterations, so the row-local
@@ -118,25 +122,43 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
         const int off = tiled ? b.read_off[i] : 0;
         float eps = kEpsGpu;
         if (tiled && bp.semantics == 1)
-            eps = bp.ph2pr[min((int)b.gcp[off], 127)];
+            eps = ph2pr[min((int)b.gcp[off], 127)];
         bool ok = true, exotic = false;
         double nl = 1.0;
-        for (int r = sl; tiled && r < L; r += 16) {
-            const uint32_t w = bp.rows[off + r];
-            exotic = exotic || row_code(w) == kExoticCode;
-            const float delta = bp.ph2pr[row_iq(w)];
-            const float sigma = bp.ph2pr[row_dq(w)];
+        // rows base+sl in steps of 16 (same trip count on the slot's lanes);
+        // the row above comes from the next lower lane, or for lane 0 from
+        // lane 15 of the previous step
+        float pdelta = 0.f, palpha = 1.f;  // row base+sl-1 (lane 0: carried)
+        uint32_t w_next = (tiled && sl < L) ? bp.rows[off + sl] : 0u;
+        for (int base = 0; tiled && base < L; base += 16) {
+            const int r = base + sl;
+            const bool act = r < L;
+            const uint32_t w = w_next;  // loaded one step ahead
+            w_next = r + 16 < L ? bp.rows[off + r + 16] : 0u;
+            const float delta = ph2pr[row_iq(w)];
+            const float sigma = ph2pr[row_dq(w)];
             const float alpha = 1.0f - (delta + sigma);
-            if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f)) {
-                ok = false;
-            } else if (r > 0) {
-                const uint32_t wp = bp.rows[off + r - 1];
-                const float pdelta = bp.ph2pr[row_iq(wp)];
-                const float palpha = 1.0f - (pdelta + bp.ph2pr[row_dq(wp)]);
-                if (!(eps * pdelta / (delta * palpha) <= 0.5f))
-                    ok = false;
+            const float up_delta = __shfl_up(delta, 1, 16);
+            const float up_alpha = __shfl_up(alpha, 1, 16);
+            if (sl > 0) {
+                pdelta = up_delta;
+                palpha = up_alpha;
             }
-            nl *= (double)alpha;
+            if (act) {
+                exotic = exotic || row_code(w) == kExoticCode;
+                if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
+                    ok = false;
+                else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
+                    ok = false;
+                nl *= (double)alpha;
+            }
+            // lane 0 of the next step needs this step's last row
+            const float last_delta = __shfl(delta, 15, 16);
+            const float last_alpha = __shfl(alpha, 15, 16);
+            if (sl == 0) {
+                pdelta = last_delta;
+                palpha = last_alpha;
+            }
         }
         // reductions over the 16 lanes of this read (xor offsets < 16 stay
         // inside the 16-lane slot)
