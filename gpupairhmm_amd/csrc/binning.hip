@@ -18,13 +18,19 @@
 
 namespace phmm {
 
-
-
-__global__ void k_clear(int32_t* p, int64_t n)
+// Several counter arrays zeroed by one launch (a fill kernel rather than
+// memsets, so the sequence can be captured and replayed as a graph).
+struct ClearList {
+    int32_t* p[6];
+    int32_t n[6];
+    int32_t count;
+};
+__global__ void k_clear_list(ClearList cl)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < cl.count; ++k)
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cl.n[k]; i += stride)
+            cl.p[k][i] = 0;
 }
 
 // Grid-stride loops throughout: a launch covers any batch size, so the
@@ -791,13 +797,21 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         int64_t b = (n + per_block - 1) / per_block;
         return (int)(b < cap ? (b < 1 ? 1 : b) : cap);
     };
-    hipLaunchKernelGGL(k_clear, dim3(grid((int64_t)bp.NW * kHapBuckets, tpb, 512)), dim3(tpb), 0,
-                       st, bp.key_count, (int64_t)bp.NW * kHapBuckets);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.hmax, (int64_t)1);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(128), 0, st, bp.rescue_counts, (int64_t)bp.NB);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.final_count, (int64_t)1);
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, bp.task_ctr, (int64_t)kTaskCounters);
-    hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, st, bp.align_part, (int64_t)kAlignParts * bp.NB);
+    ClearList cl{};
+    cl.p[0] = bp.key_count;
+    cl.n[0] = bp.NW * kHapBuckets;
+    cl.p[1] = bp.hmax;
+    cl.n[1] = 1;
+    cl.p[2] = bp.rescue_counts;
+    cl.n[2] = bp.NB;
+    cl.p[3] = bp.final_count;
+    cl.n[3] = 1;
+    cl.p[4] = bp.task_ctr;
+    cl.n[4] = kTaskCounters;
+    cl.p[5] = bp.align_part;
+    cl.n[5] = kAlignParts * bp.NB;
+    cl.count = 6;
+    hipLaunchKernelGGL(k_clear_list, dim3(512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
         hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.hap_bytes > 0)
