@@ -812,11 +812,14 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
         const Span& x = sp[(size_t)k];
         const Base& bs = base[(size_t)k];
         const size_t rb = (size_t)(x.rb1 - x.rb0), hb = (size_t)(x.hb1 - x.hb0);
+        // gap-continuation qualities are read only under CPU semantics (the
+        // GPU kernel's eps is fixed): the other semantics skip their copy
+        const size_t gb = opt.semantics == PHMM_SEM_CPU ? rb : 0;
         const Copy whole[6] = {{P + L.read_data + bs.rb, v.read_data + x.rb0, rb},
                                {P + L.bq + bs.rb, v.base_quals + x.rb0, rb},
                                {P + L.iq + bs.rb, v.ins_quals + x.rb0, rb},
                                {P + L.dq + bs.rb, v.del_quals + x.rb0, rb},
-                               {P + L.gq + bs.rb, v.gcp_quals + x.rb0, rb},
+                               {P + L.gq + bs.rb, v.gcp_quals + x.rb0, gb},
                                {P + L.hap_data + bs.hb, v.hap_data + x.hb0, hb}};
         constexpr size_t kSlice = (size_t)2 << 20;
         for (const Copy& c : whole)
@@ -844,7 +847,7 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
     };
     const size_t bulk = 5 * (size_t)rbytes + (size_t)hbytes;
     const unsigned hw = std::thread::hardware_concurrency();
-    const int nthreads = bulk >= ((size_t)16 << 20) ? (int)std::min(8u, hw > 1 ? hw : 1u) : 1;
+    const int nthreads = bulk >= ((size_t)16 << 20) ? (int)std::min(16u, hw > 1 ? hw : 1u) : 1;
     if (nthreads <= 1) {
         for (const Copy& c : copies)
             std::memcpy(c.dst, c.src, c.n);
@@ -873,8 +876,21 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
             (rc = check(hipEventCreateWithFlags(&s.compute_done, hipEventDisableTiming), "event")))
             return rc;
     }
-    if ((rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.total, hipMemcpyHostToDevice, copy_stream),
-                    "H2D copy")) ||
+    // one copy of the staged chunk; without CPU semantics the gcp region
+    // (between the deletion qualities and the haps) stays behind
+    if (opt.semantics == PHMM_SEM_CPU) {
+        rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.total, hipMemcpyHostToDevice, copy_stream),
+                   "H2D copy");
+    } else {
+        rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.gq, hipMemcpyHostToDevice, copy_stream),
+                   "H2D copy");
+        if (!rc)
+            rc = check(hipMemcpyAsync(static_cast<char*>(s.dev.p) + L.hap_data,
+                                      static_cast<char*>(s.pin.p) + L.hap_data,
+                                      L.total - L.hap_data, hipMemcpyHostToDevice, copy_stream),
+                       "H2D copy");
+    }
+    if (rc ||
         (rc = check(hipEventRecord(s.h2d_done, copy_stream), "event")) ||
         (rc = check(hipStreamWaitEvent(stream, s.h2d_done, 0), "event wait")))
         return rc;
