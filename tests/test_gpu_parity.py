@@ -123,6 +123,35 @@ def test_edge_cases(engine, oracle):
     assert st.multitile_alignments >= 0
 
 
+def test_record_with_many_haps(engine, plain_engine, oracle):
+    """One record with more haps than the binning keeps in LDS (700 > 512):
+    hap segments, item runs and -inf runs then read hap lengths from device
+    memory; some haps are empty or hold a wildcard."""
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(9)
+    rs = lambda n: "".join(rng.choice(list("ACGT"), n)) if n else ""
+    reads = [rs(int(rng.integers(100, 161))) for _ in range(19)]
+    haps = []
+    for h in range(700):
+        n = int(rng.integers(10, 61))
+        seq = rs(n)
+        if h % 97 == 5:
+            seq = ""
+        elif h % 53 == 7:
+            seq = seq[:n // 2] + "N" + seq[n // 2 + 1:]
+        haps.append(seq)
+    b = G.HostBatch.parse(_record(reads, haps))
+    say(f"case: one record, 19 reads x 700 haps, {b.num_alignments} alignments")
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    got = engine.score_batch(b)
+    assert _max_delta(got, ref) <= TOL
+    with G.Engine(small_batch=0) as e2:
+        tuned = e2.score_batch(b)
+    plain = plain_engine.score_batch(b)
+    assert np.array_equal(got.view(np.uint32), tuned.view(np.uint32))
+    assert np.array_equal(got.view(np.uint32), plain.view(np.uint32))
+
+
 def test_empty_reads_and_haps(engine):
     import gpupairhmm_amd as G
     text = "2 2\nACGT IIII IIIN IIIN ++++\n    \nACGTAC\n\n"
