@@ -15,7 +15,11 @@
 // first lane injects the row-0 boundary.  Emission probabilities for the
 // 4 bases x NR rows of each lane sit in LDS (one ds_read_b128 per 4 cells,
 // conflict-free by lane interleaving; haps holding any other byte go to the
-// one-wave kernel).  Two arithmetic forms:
+// one-wave kernel).  Its work comes in two units: super-tasks (up to 64/G
+// reads of one record against a segment of the record's haps: the wave sets
+// the reads up once and sweeps the haps one after another) and items (one
+// alignment per group: reads a record leaves over after full chunks, and the
+// FP64 rescue lists).  Two arithmetic forms:
 //
 //  SAFE (6 VALU/cell): variables M, Ib = beta*I, Dh = D/sigma_r.
 //    M  = lambda*(alpha M' + Ib' + beta sigma_{r-1} Dh')      fma fma mul
