@@ -241,43 +241,6 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         bp.align_off[G] = carry;
 }
 
-// Per group (grid-stride over groups, one block each): count its reads per
-// bin, lay them out bin by bin.
-__global__ __launch_bounds__(256) void k_group_bins(BinParams bp)
-{
-    const BatchDesc& b = *bp.bd;
-    __shared__ int cnt[kMaxBins + 1];
-    __shared__ int pos[kMaxBins + 1];
-    const int NB1 = bp.NB + 1;  // slot NB: empty reads
-    for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
-        for (int k = threadIdx.x; k < NB1; k += blockDim.x)
-            cnt[k] = 0;
-        __syncthreads();
-        const int r0 = b.rprefix[g], nr = b.nreads[g];
-        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-            const int k = bp.meta[r0 + r].bin;
-            atomicAdd(&cnt[k < 0 ? bp.NB : k], 1);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int s = 0;
-            for (int k = 0; k < NB1; ++k) {
-                pos[k] = s;
-                bp.gb_start[(int64_t)g * NB1 + k] = s;
-                bp.gb_count[(int64_t)g * NB1 + k] = cnt[k];
-                s += cnt[k];
-            }
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-            const int k = bp.meta[r0 + r].bin;
-            const int at = atomicAdd(&pos[k < 0 ? bp.NB : k], 1);
-            bp.read_perm[r0 + at] = r0 + r;
-        }
-        __syncthreads();
-    }
-}
-
 __device__ __forceinline__ int key_of(int bin, int hl)
 {
     return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
@@ -334,6 +297,50 @@ __device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, G
             gb.n = base + __popcll(m);
         __syncthreads();
     }
+}
+
+// One wave per group: count the group's reads per bin (slot NB: empty
+// reads), store the counts, starts and the bin-ordered read permutation for
+// the scatter, and keep the non-empty bins in LDS (as group_bins_compact
+// would read them back).
+__device__ __forceinline__ void group_bins_build(const BinParams& bp, const BatchDesc& b, int g,
+                                                 GroupBins& gb, int* cnt, int* pos)
+{
+    const int lane = threadIdx.x;
+    const int NB1 = bp.NB + 1;
+    for (int k = lane; k < NB1; k += 64)
+        cnt[k] = 0;
+    __syncthreads();
+    const int r0 = b.rprefix[g], nr = b.nreads[g];
+    for (int r = lane; r < nr; r += 64) {
+        const int k = bp.meta[r0 + r].bin;
+        atomicAdd(&cnt[k < 0 ? bp.NB : k], 1);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int s = 0, n = 0;
+        for (int k = 0; k < NB1; ++k) {
+            const int c = cnt[k];
+            pos[k] = s;
+            bp.gb_start[(int64_t)g * NB1 + k] = s;
+            bp.gb_count[(int64_t)g * NB1 + k] = c;
+            if (c > 0) {
+                gb.k[n] = k;
+                gb.cnt[n] = c;
+                gb.st[n] = s;
+                ++n;
+            }
+            s += c;
+        }
+        gb.n = n;
+    }
+    __syncthreads();
+    for (int r = lane; r < nr; r += 64) {
+        const int k = bp.meta[r0 + r].bin;
+        const int at = atomicAdd(&pos[k < 0 ? bp.NB : k], 1);
+        bp.read_perm[r0 + at] = r0 + r;
+    }
+    __syncthreads();
 }
 
 // Per-group hap summary, uniform across the wave: haps with a sequence and
@@ -451,18 +458,20 @@ __device__ __forceinline__ ChunkPlan chunk_plan(const BinParams& bp, int k, int 
 // The item bin of tile bin k (work-bin ids past the multi-tile and exact bins).
 __device__ __forceinline__ int item_bin_of(const BinParams& bp, int k) { return bp.NB + k; }
 
-// Per group (one wave): the sort-key histogram and the per-bin alignment
-// counts.  A tile bin gets one key (bin, total hap length) per group, counted
-// in super-tasks; the multi-tile and exact bins get one key (bin, hap length)
+// Per group (one wave): its reads per bin (group_bins_build), the per-bin
+// alignment counts and the sort-key histogram.  A tile bin's full chunks get
+// one key (bin, segment columns) per hap segment, counted in super-tasks;
+// item bins, the multi-tile and the exact bins get one key (bin, hap length)
 // per (hap, bin) run, counted in alignments.
 __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     __shared__ GroupBins gb;
     __shared__ GroupHaps gh;
+    __shared__ int cnt[kMaxBins + 1], pos[kMaxBins + 1];
     const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
-        group_bins_compact(bp, g, gb);
+        group_bins_build(bp, b, g, gb, cnt, pos);
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
@@ -824,7 +833,6 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_hap_flags, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
-        hipLaunchKernelGGL(k_group_bins, dim3(grid(hd.G, 1, 8192)), dim3(256), 0, st, bp);
         hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
     }
     hipLaunchKernelGGL(k_scan_keys, dim3(bp.NW), dim3(1024), 0, st, bp);
