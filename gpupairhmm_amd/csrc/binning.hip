@@ -66,32 +66,54 @@ __global__ void k_encode_reads(BinParams bp)
     }
 }
 
-__global__ void k_encode_haps(BinParams bp)
+// One wave per hap: its code bytes (dwords over the padded length), whether
+// it holds a byte other than A/C/G/T within its length (the wildcard --
+// reference GPU semantics: any such byte; CPU semantics: 'N' -- or, under
+// CPU semantics, a byte that matches nothing but itself; alignments with
+// such haps go to the exact bin, so the tile kernels' emission tables need
+// only the four bases), and the longest hap (per-block maxima into
+// kHmaxParts counters, folded by k_scan_bins).
+__global__ __launch_bounds__(256) void k_hap_pass(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
+    __shared__ int wmax[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
     const uint32_t* hd = reinterpret_cast<const uint32_t*>(b.hap_data);
     uint32_t* out = reinterpret_cast<uint32_t*>(bp.hapc);
-    GRID_STRIDE(i, b.hap_bytes >> 2)
-    {
-        const uint32_t h = hd[i];
-        uint32_t w = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            w |= (uint32_t)base_code_sem((uint8_t)(h >> (8 * k)), bp.semantics) << (8 * k);
-        out[i] = w;
-    }
-}
-
-__global__ void k_hap_max(BinParams bp)
-{
-    const BatchDesc& b = *bp.bd;
     int m = 0;
-    GRID_STRIDE(i, b.nH)
-    m = max(m, b.hap_len[i]);
+    for (int h = blockIdx.x * (blockDim.x >> 6) + wv; h < b.nH; h += nwaves) {
+        const int hl = b.hap_len[h], d0 = b.hap_off[h] >> 2;
+        m = max(m, hl);
+        const int nd = (max(hl, 0) >> 2) + 1;  // padded length: hl + 4 - hl % 4
+        bool other = false;
+        for (int d = lane; d < nd; d += 64) {
+            const uint32_t w = hd[d0 + d];
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t code = (uint32_t)base_code_sem((uint8_t)(w >> (8 * k)), bp.semantics);
+                c |= code << (8 * k);
+                other = other || (4 * d + k < hl && code >= 4u);
+            }
+            out[d0 + d] = c;
+        }
+        other = __any(other);
+        if (lane == 0)
+            bp.hapflag[h] = other ? 1 : 0;
+    }
+#pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         m = max(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(bp.hmax, m);
+    if (lane == 0)
+        wmax[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bm = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k)
+            bm = max(bm, wmax[k]);
+        atomicMax(&bp.hmax_part[blockIdx.x & (kHmaxParts - 1)], bm);
+    }
 }
 
 // Per read (16 lanes each, four reads per wave): its bin.  Tile class k is
@@ -245,28 +267,6 @@ __device__ __forceinline__ int key_of(int bin, int hl)
 {
     return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
 }
-
-// Per hap (one wave each), whether it holds a byte other than A/C/G/T: the
-// wildcard (reference GPU semantics: any such byte; CPU semantics: 'N') or,
-// under CPU semantics, a byte that matches nothing but itself.  Alignments
-// with such haps go to the exact bin, so the tile kernels' emission tables
-// need only the four bases.
-__global__ __launch_bounds__(256) void k_hap_flags(BinParams bp)
-{
-    const BatchDesc& b = *bp.bd;
-    const int lane = threadIdx.x & 63;
-    const int nwaves = gridDim.x * (blockDim.x >> 6);
-    for (int h = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); h < b.nH; h += nwaves) {
-        const int hl = b.hap_len[h], off = b.hap_off[h];
-        bool other = false;
-        for (int c = lane; c < hl; c += 64)
-            other = other || bp.hapc[off + c] >= 4;
-        other = __any(other);
-        if (lane == 0)
-            bp.hapflag[h] = other ? 1 : 0;
-    }
-}
-
 
 // A group's non-empty read bins (including the empty-read slot NB), built
 // by one wave in LDS: bin id, count and start in the group's read_perm.
@@ -601,6 +601,14 @@ __device__ __forceinline__ void scan_pairs(int32_t* range, int NB, const int* cn
 __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 {
     __shared__ int tot[kMaxBins];
+    {  // the longest hap, from the hap pass's per-block maxima
+        int m = (int)threadIdx.x < kHmaxParts ? bp.hmax_part[threadIdx.x] : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            m = max(m, __shfl_xor(m, o, 64));
+        if (threadIdx.x == 0)
+            *bp.hmax = m;
+    }
     scan_pairs(bp.bin_range, bp.NW, nullptr);
     for (int k = threadIdx.x; k < bp.NB; k += 64) {  // each bin's alignment count from its parts
         int v = 0;
@@ -809,8 +817,8 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     ClearList cl{};
     cl.p[0] = bp.key_count;
     cl.n[0] = bp.NW * kHapBuckets;
-    cl.p[1] = bp.hmax;
-    cl.n[1] = 1;
+    cl.p[1] = bp.hmax_part;
+    cl.n[1] = kHmaxParts;
     cl.p[2] = bp.rescue_counts;
     cl.n[2] = bp.NB;
     cl.p[3] = bp.final_count;
@@ -823,14 +831,10 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     hipLaunchKernelGGL(k_clear_list, dim3(512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
         hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.hap_bytes > 0)
-        hipLaunchKernelGGL(k_encode_haps, dim3(grid(hd.hap_bytes >> 2, tpb, 2048)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.nH > 0)
-        hipLaunchKernelGGL(k_hap_max, dim3(grid(hd.nH, tpb, 256)), dim3(tpb), 0, st, bp);
+    if (fixed_grids || hd.nH > 0)  // one wave per hap
+        hipLaunchKernelGGL(k_hap_pass, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     if (fixed_grids || hd.nR > 0)  // 16 lanes per read
         hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 16, 4096)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.nH > 0)  // one wave per hap
-        hipLaunchKernelGGL(k_hap_flags, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
         hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);

@@ -294,7 +294,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
 
     // counters: [0] max hap length, [1] final-list count, [2] error flag,
     // [4 .. 4+NB) per-bin rescue counts, [128 .. 128+kTaskCounters) task
-    // counters of the set launches
+    // counters of the set launches, [224 .. 224+kHmaxParts) partial maxima
+    static_assert(128 + kTaskCounters <= 224 && 224 + kHmaxParts <= 256, "counter layout");
     int32_t* sm = static_cast<int32_t*>(smallints.p);
     BinParams bp{};
     bp.bd = static_cast<const BatchDesc*>(desc.p);
@@ -311,6 +312,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
     bp.bin_range = static_cast<int32_t*>(bin_range.p);
     bp.hmax = sm + 0;
+    bp.hmax_part = sm + 224;
     bp.final_count = sm + 1;
     bp.error_flag = sm + 2;
     bp.rescue_counts = sm + 4;

@@ -66,6 +66,7 @@ constexpr int kTaskCounters = 96;   // set launches per call (plain mode: one pe
 constexpr int kF64Sets = 8;         // of them, the last ones: FP64 tile sets
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 constexpr int kAlignParts = 64;     // partial counters per bin (atomic spreading)
+constexpr int kHmaxParts = 32;      // partial maxima of the hap pass (atomic spreading)
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
 __host__ __device__ inline int hap_bucket(int h)
@@ -170,7 +171,8 @@ struct BinParams {
     int32_t* key_count;        // [NW*kHapBuckets]
     int32_t* key_cursor;       // [NW*kHapBuckets]
     int32_t* bin_range;        // [2*NW]: begin, count (tile bins: super-tasks; others: items)
-    int32_t* hmax;             // [1]
+    int32_t* hmax;             // [1]: the longest hap (k_scan_bins folds hmax_part)
+    int32_t* hmax_part;        // [kHmaxParts]: per-block maxima of the hap pass
     int32_t* rescue_counts;    // [NB]
     int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
