@@ -98,7 +98,7 @@ __device__ __forceinline__ float ph2pr_of(const float* table, int q)
 }
 
 // ---------------------------------------------------------------------------
-// k_fwd_tile
+// k_fwd_tile_set
 // ---------------------------------------------------------------------------
 template <typename T>
 struct TileTraits;

@@ -38,7 +38,7 @@ constexpr float kLog10Scale = 37.3277194f;      // log10(FLT_MAX / 16)
 constexpr double kScaleD = 2.1267646664908054e37;
 
 // Headroom divisor of the rescaled kernel's start constant (see
-// k_fwd_tile): every rescaled cell stays below FLT_MAX/16.
+// k_fwd_tile_set): every rescaled cell stays below FLT_MAX/16.
 constexpr float kFastKappa = 2.0f;
 constexpr float kLog10FastKappa = 0.30103000f;
 
