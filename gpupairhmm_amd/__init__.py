@@ -52,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "phmm_synth_generate", "phmm_host_batch_to_text", "phmm_format_scores", "phmm_free_text",
     "phmm_batch_cells", "phmm_server_options_default", "phmm_server_create",
     "phmm_server_destroy", "phmm_server_submit", "phmm_server_poll", "phmm_server_stats_get",
+    "phmm_compute_likelihoods",
 )
 
 
@@ -110,6 +111,15 @@ class Stats(ctypes.Structure):
 class TileConfig(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32), ("rows", ctypes.c_int32),
                 ("max_read_len", ctypes.c_int32)]
+
+
+class RegionRead(ctypes.Structure):
+    _fields_ = [("bases", _u8p), ("base_quals", _u8p), ("ins_quals", _u8p), ("del_quals", _u8p),
+                ("gcp_quals", _u8p), ("length", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class RegionHap(ctypes.Structure):
+    _fields_ = [("bases", _u8p), ("length", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -174,6 +184,9 @@ def load_library():
         "phmm_score_batch": (ctypes.c_int, [vp, ctypes.POINTER(BatchView),
                                             ctypes.POINTER(ctypes.c_float)]),
         "phmm_score_device": (ctypes.c_int, [vp, ctypes.POINTER(BatchView), vp, vp]),
+        "phmm_compute_likelihoods": (ctypes.c_int, [vp, ctypes.POINTER(RegionRead), ctypes.c_int32,
+                                                    ctypes.POINTER(RegionHap), ctypes.c_int32,
+                                                    ctypes.POINTER(ctypes.c_double)]),
         "phmm_engine_synchronize": (ctypes.c_int, [vp]),
         "phmm_device_batch_upload": (ctypes.c_int, [vp, ctypes.POINTER(BatchView),
                                                     ctypes.POINTER(vp)]),
@@ -532,6 +545,38 @@ class Engine:
         out = np.empty(batch.num_alignments, dtype=np.float32)
         self._chk(load_library().phmm_score_batch(
             self._h, ctypes.byref(batch.view), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def compute_likelihoods(self, reads, haps) -> np.ndarray:
+        """One region's log10 likelihood matrix, shape (len(reads), len(haps)).
+
+        reads: sequence of (bases, base_quals, ins_quals, del_quals, gcp_quals),
+        bases as bytes/str and qualities as phred values (not ASCII); haps:
+        sequence of bytes/str."""
+        def u8(x):
+            if isinstance(x, str):
+                x = x.encode()
+            return np.ascontiguousarray(np.frombuffer(x, dtype=np.uint8) if isinstance(x, bytes)
+                                        else np.asarray(x, dtype=np.uint8))
+        keep = []
+        rr = (RegionRead * max(len(reads), 1))()
+        for i, (bases, bq, iq, dq, gq) in enumerate(reads):
+            arrs = [u8(bases), u8(bq), u8(iq), u8(dq), u8(gq)]
+            L = len(arrs[0])
+            if any(len(a) != L for a in arrs[1:]):
+                raise ValueError(f"read {i}: qualities must match the read length")
+            keep.append(arrs)
+            ptrs = [a.ctypes.data_as(_u8p) for a in arrs]
+            rr[i] = RegionRead(*ptrs, L, 0)
+        hh = (RegionHap * max(len(haps), 1))()
+        for i, h in enumerate(haps):
+            a = u8(h)
+            keep.append(a)
+            hh[i] = RegionHap(a.ctypes.data_as(_u8p), len(a), 0)
+        out = np.empty((len(reads), len(haps)), dtype=np.float64)
+        self._chk(load_library().phmm_compute_likelihoods(
+            self._h, rr, len(reads), hh, len(haps),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
     def upload(self, batch: HostBatch) -> DeviceBatch:

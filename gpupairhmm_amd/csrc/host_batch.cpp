@@ -446,3 +446,62 @@ extern "C" int64_t phmm_batch_cells(const phmm_batch_view* v)
     }
     return total;
 }
+
+// One region (reads x haps with phred qualities) as one padded record of a
+// host batch, scored by phmm_score_batch; log10 likelihoods widened to
+// double for callers that keep them so.
+extern "C" int phmm_compute_likelihoods(phmm_engine* eng, const phmm_region_read* reads,
+                                        int32_t num_reads, const phmm_region_hap* haps,
+                                        int32_t num_haps, double* out)
+{
+    if (!eng || num_reads < 0 || num_haps < 0 || (num_reads > 0 && !reads) ||
+        (num_haps > 0 && !haps) || (num_reads > 0 && num_haps > 0 && !out))
+        return phmm::fail(PHMM_EINVAL, "phmm_compute_likelihoods: bad arguments");
+    if (num_reads == 0 || num_haps == 0)
+        return PHMM_OK;
+    constexpr uint8_t pad_char = 'N';
+    constexpr uint8_t pad_qual = 'N' - 33;  // the parser's padding, as a phred value
+    phmm::SequenceGroup g;
+    g.num_reads = num_reads;
+    g.num_haps = num_haps;
+    for (int32_t i = 0; i < num_reads; ++i) {
+        const phmm_region_read& r = reads[i];
+        if (r.length < 0 || (r.length > 0 && (!r.bases || !r.base_quals || !r.ins_quals ||
+                                              !r.del_quals || !r.gcp_quals)))
+            return phmm::fail(PHMM_EINVAL, "phmm_compute_likelihoods: read " + std::to_string(i) +
+                                               ": negative length or null array");
+        const int pad = phmm::padding_of(r.length);
+        g.read_data.insert(g.read_data.end(), r.bases, r.bases + r.length);
+        g.read_data.insert(g.read_data.end(), pad, pad_char);
+        const uint8_t* q[4] = {r.base_quals, r.ins_quals, r.del_quals, r.gcp_quals};
+        std::vector<uint8_t>* qv[4] = {&g.base_quals, &g.ins_quals, &g.del_quals, &g.gcp_quals};
+        for (int k = 0; k < 4; ++k) {
+            qv[k]->insert(qv[k]->end(), q[k], q[k] + r.length);
+            qv[k]->insert(qv[k]->end(), pad, pad_qual);
+        }
+        g.read_lengths.push_back(r.length);
+    }
+    for (int32_t i = 0; i < num_haps; ++i) {
+        const phmm_region_hap& h = haps[i];
+        if (h.length < 0 || (h.length > 0 && !h.bases))
+            return phmm::fail(PHMM_EINVAL, "phmm_compute_likelihoods: hap " + std::to_string(i) +
+                                               ": negative length or null bases");
+        g.hap_data.insert(g.hap_data.end(), h.bases, h.bases + h.length);
+        g.hap_data.insert(g.hap_data.end(), phmm::padding_of(h.length), pad_char);
+        g.hap_lengths.push_back(h.length);
+    }
+    phmm::HostBatch hb;
+    try {
+        hb.assign({g});
+    } catch (const std::exception& e) {
+        return phmm::fail(PHMM_ENOMEM, std::string("phmm_compute_likelihoods: ") + e.what());
+    }
+    const phmm_batch_view v = hb.view();
+    std::vector<float> scores((size_t)num_reads * (size_t)num_haps);
+    const int rc = phmm_score_batch(eng, &v, scores.data());
+    if (rc != PHMM_OK)
+        return rc;
+    for (size_t i = 0; i < scores.size(); ++i)
+        out[i] = (double)scores[i];
+    return PHMM_OK;
+}

@@ -173,6 +173,32 @@ int phmm_engine_get_tiles(const phmm_engine* eng, phmm_tile_config* tiles, int c
  * gpuworker.cuh:559-973) */
 int phmm_score_batch(phmm_engine* eng, const phmm_batch_view* batch, float* scores_out);
 
+/* ---- in-process callers: one region's likelihood matrix --------------- *
+ * What a variant caller's per-region PairHMM call computes (GATK's
+ * PairHMM.computeLikelihoods, GKL's computeLikelihoodsNative): R reads, each
+ * with base, insertion, deletion and gap-continuation qualities as phred
+ * values (not ASCII), against H haplotypes; out[r*H + h] receives the log10
+ * likelihood of read r given hap h, as phmm_score_batch scores the same
+ * region as one record.  No counterpart in the reference, whose callers go
+ * through files; this is the adapter such a caller binds (over JNI, ctypes,
+ * ...) instead of writing records.  Synchronous. */
+typedef struct phmm_region_read {
+    const uint8_t* bases;
+    const uint8_t* base_quals;
+    const uint8_t* ins_quals;
+    const uint8_t* del_quals;
+    const uint8_t* gcp_quals;
+    int32_t length;
+    int32_t reserved;
+} phmm_region_read;
+typedef struct phmm_region_hap {
+    const uint8_t* bases;
+    int32_t length;
+    int32_t reserved;
+} phmm_region_hap;
+int phmm_compute_likelihoods(phmm_engine* eng, const phmm_region_read* reads, int32_t num_reads,
+                             const phmm_region_hap* haps, int32_t num_haps, double* out);
+
 /* Device batch in (array pointers are device memory), device scores out,
  * ordered on `hip_stream` (a hipStream_t; NULL = the engine's stream).  In
  * PHMM_MODE_BATCHED the call only enqueues work; in PHMM_MODE_PLAIN it waits

@@ -50,6 +50,14 @@ def test_struct_sizes_match_header(lib):
     assert ctypes.sizeof(G.TileConfig) == 12
 
 
+def test_region_likelihoods_reject_bad_arguments(lib):
+    rr = (G.RegionRead * 1)()
+    hh = (G.RegionHap * 1)()
+    out = (ctypes.c_double * 1)()
+    assert lib.phmm_compute_likelihoods(None, rr, 1, hh, 1, out) == G.EINVAL
+    assert b"phmm_compute_likelihoods" in lib.phmm_last_error()
+
+
 def test_null_arguments_are_rejected(lib):
     assert lib.phmm_engine_create(0, None, None) == G.EINVAL
     assert lib.phmm_score_batch(None, None, None) == G.EINVAL

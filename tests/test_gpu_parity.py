@@ -152,6 +152,31 @@ def test_record_with_many_haps(engine, plain_engine, oracle):
     assert np.array_equal(got.view(np.uint32), plain.view(np.uint32))
 
 
+def test_region_likelihoods_match_record_scoring(engine, oracle):
+    """The in-process adapter (reads with phred qualities x haps in, a
+    log10 likelihood matrix out) gives exactly the scores of the same region
+    written as one record, and those match the oracle."""
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(21)
+    rs = lambda n: "".join(rng.choice(list("ACGTN"), n, p=[.24, .24, .24, .24, .04]))
+    reads, lines = [], []
+    for _ in range(7):
+        L = int(rng.integers(1, 220))
+        q = [rng.integers(5, 45, L) for _ in range(4)]
+        bases = rs(L)
+        reads.append((bases, *q))
+        lines.append(" ".join([bases] + ["".join(chr(33 + int(x)) for x in a) for a in q]))
+    haps = [rs(int(rng.integers(1, 400))) for _ in range(5)]
+    say("case: region likelihoods, 7 reads x 5 haps")
+    got = engine.compute_likelihoods(reads, haps)
+    b = G.HostBatch.parse("\n".join([f"{len(reads)} {len(haps)}"] + lines + haps) + "\n")
+    ref32 = engine.score_batch(b).astype(np.float64).reshape(len(reads), len(haps))
+    assert got.shape == (7, 5)
+    assert np.array_equal(got, ref32)
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64).reshape(len(reads), len(haps))
+    assert _max_delta(got.ravel(), ref.ravel()) <= TOL
+
+
 def test_empty_reads_and_haps(engine):
     import gpupairhmm_amd as G
     text = "2 2\nACGT IIII IIIN IIIN ++++\n    \nACGTAC\n\n"
