@@ -25,16 +25,20 @@ def _bits(x):
     return np.asarray(x, dtype=np.float32).view(np.uint32)
 
 
-@pytest.mark.parametrize("timed", [False, True])
-def test_graph_replay_matches_eager_and_plain(lib, timed):
+# small_batch: None = default (every batch here on the one-template table),
+# 0 = the tuned table throughout, 1000 = calls switch between the two
+@pytest.mark.parametrize("timed,small", [(False, None), (True, None), (False, 0), (False, 1000)])
+def test_graph_replay_matches_eager_and_plain(lib, timed, small):
     import gpupairhmm_amd as G
     bs = _batches()
     with G.Engine(mode=G.MODE_PLAIN) as plain, \
-            G.Engine(mode=G.MODE_BATCHED, graphs=False) as eager, \
-            G.Engine(mode=G.MODE_BATCHED, graphs=True, time_kernels=timed) as graph:
+            G.Engine(mode=G.MODE_BATCHED, graphs=False, small_batch=small) as eager, \
+            G.Engine(mode=G.MODE_BATCHED, graphs=True, time_kernels=timed,
+                     small_batch=small) as graph:
         for rep in range(2):
             for i, b in enumerate(bs):
-                say(f"case: graphs timed={timed} rep {rep} batch {i} ({b.num_alignments} alignments)")
+                say(f"case: graphs timed={timed} small_batch={small} rep {rep} batch {i} "
+                    f"({b.num_alignments} alignments)")
                 want = plain.score_batch(b)
                 assert np.array_equal(_bits(eager.score_batch(b)), _bits(want))
                 assert np.array_equal(_bits(graph.score_batch(b)), _bits(want))
