@@ -40,32 +40,8 @@ __global__ void k_clear_list(ClearList cl)
          i += (int64_t)gridDim.x * blockDim.x)
 
 // Sequence bytes are padded to multiples of 4 per read/hap (the reference's
-// padding rule), so both encoders work on 4 bytes per thread: dword loads of
-// each input array, one 16-byte (reads) or 4-byte (haps) store.
-__global__ void k_encode_reads(BinParams bp)
-{
-    const BatchDesc& b = *bp.bd;
-    const uint32_t* rd = reinterpret_cast<const uint32_t*>(b.read_data);
-    const uint32_t* q0 = reinterpret_cast<const uint32_t*>(b.bq);
-    const uint32_t* q1 = reinterpret_cast<const uint32_t*>(b.iq);
-    const uint32_t* q2 = reinterpret_cast<const uint32_t*>(b.dq);
-    uint4* out = reinterpret_cast<uint4*>(bp.rows);
-    GRID_STRIDE(i, b.read_bytes >> 2)
-    {
-        const uint32_t r = rd[i], a = q0[i], c = q1[i], d = q2[i];
-        uint32_t w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t code = (uint32_t)base_code_sem((uint8_t)(r >> (8 * k)), bp.semantics);
-            const uint32_t qb = min((a >> (8 * k)) & 0xffu, 127u);
-            const uint32_t qi = min((c >> (8 * k)) & 0xffu, 127u);
-            const uint32_t qd = min((d >> (8 * k)) & 0xffu, 127u);
-            w[k] = code | (qb << 8) | (qi << 16) | (qd << 24);
-        }
-        out[i] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-}
-
+// padding rule) and start 4-byte aligned, so both passes below load and
+// encode 4 bytes per lane: dword loads of each input array.
 // One wave per hap: its code bytes (dwords over the padded length), whether
 // it holds a byte other than A/C/G/T within its length (the wildcard --
 // reference GPU semantics: any such byte; CPU semantics: 'N' -- or, under
@@ -112,18 +88,23 @@ __global__ __launch_bounds__(256) void k_hap_pass(BinParams bp)
         int bm = 0;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k)
             bm = max(bm, wmax[k]);
-        atomicMax(&bp.hmax_part[blockIdx.x & (kHmaxParts - 1)], bm);
+        if (bm > 0)  // the parts start at 0: blocks without haps skip the atomic
+            atomicMax(&bp.hmax_part[blockIdx.x & (kHmaxParts - 1)], bm);
     }
 }
 
-// Per read (16 lanes each, four reads per wave): its bin.  Tile class k is
-// the first table row whose max_read_len covers the read (its tile has room
-// for one more row); reads beyond the last row go to the multi-tile bin
-// 2*ntiles.  Bin 2k uses the rescaled (FAST) arithmetic when the read's
-// coefficients keep it in range, else bin 2k+1 (SAFE).  Reference CPU
-// semantics: a read holding a byte other than A/C/G/T/N goes to the exact
-// bin 2*ntiles+1.  The 16 lanes check rows sl, sl+16, ... and combine their
-// verdicts and the product of alpha with row-local reductions.
+// Per read (16 lanes each, four reads per wave): its packed rows and its
+// bin.  Lane sl of a read's slot takes rows 4d .. 4d+3 for d = sl, sl+16,
+// ... over the read's padded length: one dword of each input array, one
+// 16-byte store of the packed words (base code | base q << 8 | ins q << 16 |
+// del q << 24, qualities clamped to 127).  Tile class k is the first table
+// row whose max_read_len covers the read (its tile has room for one more
+// row); reads beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k
+// uses the rescaled (FAST) arithmetic when the read's coefficients keep it in
+// range, else bin 2k+1 (SAFE).  Reference CPU semantics: a read holding a
+// byte other than A/C/G/T/N goes to the exact bin 2*ntiles+1.  The slot
+// combines its lanes' verdicts and the product of alpha with row-local
+// reductions.
 __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
@@ -131,6 +112,11 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
     for (int q = threadIdx.x; q < 128; q += blockDim.x)
         ph2pr[q] = bp.ph2pr[q];
     __syncthreads();
+    const uint32_t* rd = reinterpret_cast<const uint32_t*>(b.read_data);
+    const uint32_t* q0 = reinterpret_cast<const uint32_t*>(b.bq);
+    const uint32_t* q1 = reinterpret_cast<const uint32_t*>(b.iq);
+    const uint32_t* q2 = reinterpret_cast<const uint32_t*>(b.dq);
+    uint4* rows4 = reinterpret_cast<uint4*>(bp.rows);
     const int sl = threadIdx.x & 15;
     const int nslots = gridDim.x * (blockDim.x >> 4);
     // every 16-lane slot runs the same number of iterations, so the row-local
@@ -147,42 +133,64 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
                 break;
             }
         const bool tiled = valid && L > 0 && k < bp.ntiles;
-        const int off = tiled ? b.read_off[i] : 0;
+        const int off = (valid && L > 0) ? b.read_off[i] : 0;
+        const int nd = (valid && L > 0) ? (L >> 2) + 1 : 0;  // dwords of the padded read
         float eps = kEpsGpu;
         if (tiled && bp.semantics == 1)
             eps = ph2pr[min((int)b.gcp[off], 127)];
         bool ok = true, exotic = false;
         double nl = 1.0;
-        // rows base+sl in steps of 16 (same trip count on the slot's lanes);
-        // the row above comes from the next lower lane, or for lane 0 from
-        // lane 15 of the previous step
-        float pdelta = 0.f, palpha = 1.f;  // row base+sl-1 (lane 0: carried)
-        uint32_t w_next = (tiled && sl < L) ? bp.rows[off + sl] : 0u;
-        for (int base = 0; tiled && base < L; base += 16) {
-            const int r = base + sl;
-            const bool act = r < L;
-            const uint32_t w = w_next;  // loaded one step ahead
-            w_next = r + 16 < L ? bp.rows[off + r + 16] : 0u;
-            const float delta = ph2pr[row_iq(w)];
-            const float sigma = ph2pr[row_dq(w)];
-            const float alpha = 1.0f - (delta + sigma);
-            const float up_delta = __shfl_up(delta, 1, 16);
-            const float up_alpha = __shfl_up(alpha, 1, 16);
+        float pdelta = 0.f, palpha = 1.f;  // the row above this lane's first row
+        // same trip count on the slot's lanes (nd is the read's)
+        for (int base = 0; base < nd; base += 16) {
+            const int d = base + sl;
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (d < nd) {
+                const int q = (off >> 2) + d;
+                const uint32_t r = rd[q], a = q0[q], c = q1[q], e = q2[q];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    const uint32_t code = (uint32_t)base_code_sem((uint8_t)(r >> (8 * x)), bp.semantics);
+                    const uint32_t qb = min((a >> (8 * x)) & 0xffu, 127u);
+                    const uint32_t qi = min((c >> (8 * x)) & 0xffu, 127u);
+                    const uint32_t qd = min((e >> (8 * x)) & 0xffu, 127u);
+                    w[x] = code | (qb << 8) | (qi << 16) | (qd << 24);
+                }
+                rows4[q] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+            if (!tiled)  // uniform over the slot
+                continue;
+            float dl[4], al[4], sg[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                dl[x] = ph2pr[row_iq(w[x])];
+                sg[x] = ph2pr[row_dq(w[x])];
+                al[x] = 1.0f - (dl[x] + sg[x]);
+            }
+            // row 4d-1: the next lower lane's last row; lane 0 keeps the
+            // previous step's lane 15 row
+            const float up_delta = __shfl_up(dl[3], 1, 16);
+            const float up_alpha = __shfl_up(al[3], 1, 16);
             if (sl > 0) {
                 pdelta = up_delta;
                 palpha = up_alpha;
             }
-            if (act) {
-                exotic = exotic || row_code(w) == kExoticCode;
-                if (!(alpha > 0.f && alpha <= 1.f && delta > 0.f && sigma > 0.f))
-                    ok = false;
-                else if (r > 0 && !(eps * pdelta / (delta * palpha) <= 0.5f))
-                    ok = false;
-                nl *= (double)alpha;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const int r = 4 * d + x;
+                if (r < L) {
+                    exotic = exotic || row_code(w[x]) == kExoticCode;
+                    if (!(al[x] > 0.f && al[x] <= 1.f && dl[x] > 0.f && sg[x] > 0.f))
+                        ok = false;
+                    else if (r > 0 && !(eps * pdelta / (dl[x] * palpha) <= 0.5f))
+                        ok = false;
+                    nl *= (double)al[x];
+                }
+                pdelta = dl[x];
+                palpha = al[x];
             }
-            // lane 0 of the next step needs this step's last row
-            const float last_delta = __shfl(delta, 15, 16);
-            const float last_alpha = __shfl(alpha, 15, 16);
+            const float last_delta = __shfl(dl[3], 15, 16);
+            const float last_alpha = __shfl(al[3], 15, 16);
             if (sl == 0) {
                 pdelta = last_delta;
                 palpha = last_alpha;
@@ -347,7 +355,7 @@ __device__ __forceinline__ void group_bins_build(const BinParams& bp, const Batc
 // those of them that go to the exact bin (a byte other than A/C/G/T); the others are
 // cut, in order, into segments of at most `budget` columns (at least one hap
 // each) -- the super-tasks' hap ranges and their work (sort key).
-constexpr int kMaxSegs = 512;
+constexpr int kMaxSegs = 128;
 struct GroupHaps {
     int nonempty, other, nseg;
     int hs[kMaxSegs], he[kMaxSegs], work[kMaxSegs];
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
-    constexpr int kRuns = 1024;
+    constexpr int kRuns = 256;
     constexpr int kSkip = -2;  // run covered by a super-task
     __shared__ GroupBins gb;
     __shared__ GroupHaps gh;
@@ -803,14 +811,17 @@ hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStr
 }
 
 // ---------------------------------------------------------------------------
-hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, hipStream_t st)
+hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, bool small,
+                       hipStream_t st)
 {
     const int tpb = 256;
     // grids: sized to the batch, or fixed (graph capture) at a size that
-    // still fills the chip
+    // still fills the chip -- a quarter of it for the small-batch sequence,
+    // whose batches are too small to use more and whose latency pays for
+    // every idle wave dispatched
     auto grid = [&](int64_t n, int per_block, int cap) {
         if (fixed_grids)
-            return cap;
+            return small ? cap / 4 : cap;
         int64_t b = (n + per_block - 1) / per_block;
         return (int)(b < cap ? (b < 1 ? 1 : b) : cap);
     };
@@ -828,12 +839,10 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     cl.p[5] = bp.align_part;
     cl.n[5] = kAlignParts * bp.NB;
     cl.count = 6;
-    hipLaunchKernelGGL(k_clear_list, dim3(512), dim3(tpb), 0, st, cl);
-    if (fixed_grids || hd.read_bytes > 0)  // 4 rows per thread
-        hipLaunchKernelGGL(k_encode_reads, dim3(grid(hd.read_bytes >> 2, tpb, 4096)), dim3(tpb), 0, st, bp);
+    hipLaunchKernelGGL(k_clear_list, dim3(fixed_grids && small ? 128 : 512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.nH > 0)  // one wave per hap
         hipLaunchKernelGGL(k_hap_pass, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);
-    if (fixed_grids || hd.nR > 0)  // 16 lanes per read
+    if (fixed_grids || hd.nR > 0)  // 16 lanes per read: rows and bins
         hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 16, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {

@@ -425,7 +425,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
 
 int Engine::enqueue_prep(const BinParams& bp, const BatchDesc& hd, bool fixed, hipStream_t st)
 {
-    return check(run_binning(bp, hd, fixed, st), "binning kernels");
+    return check(run_binning(bp, hd, fixed, act == &tiles_small, st), "binning kernels");
 }
 
 // Pass 1: FP32 tile kernels over the tile bins' super-tasks (results below
@@ -547,8 +547,12 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     // wave it occupies (64 / alignments per wave) times the per-step work of
     // a lane (5 or 6 FP64 ops per row plus about 12 for the exchange).  The
     // 10-row FAST tile (248 VGPRs) still runs two waves per SIMD; the 10-row
-    // SAFE one would not, so SAFE stops at 8.
+    // SAFE one would not, so SAFE stops at 8.  The small-batch table takes
+    // the widest tile of each form for every bin: a small batch's FP64 time
+    // is the tail of each set launch, so one set per form beats a set per
+    // cost choice.
     static constexpr int kNr64[] = {4, 6, 8, 10};
+    const bool one_set = act == &tiles_small;
     BinSet set[4][2], wave;  // [rows index][SAFE, FAST]
     for (auto& row : set)
         for (BinSet& x : row) {
@@ -578,6 +582,8 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
             const int nr = kNr64[i];
             const int G = (tile + nr - 1) / nr;
             if (G > 64 || (!fast && nr > 8))
+                continue;
+            if (one_set && nr != (fast ? 10 : 8))
                 continue;
             const double cost = (64.0 / (64 / G)) * ((fast ? 5 : 6) * nr + 12);
             if (best < 0 || cost < best_cost) {

@@ -193,8 +193,10 @@ struct BinParams {
 
 // Binning launches.  `hd` is the host copy of the descriptor bp.bd points to;
 // with fixed_grids every grid is independent of the batch sizes (graph
-// capture), else grids are sized to them.
-hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, hipStream_t st);
+// capture; `small`: the smaller grids of the small-batch sequence), else
+// grids are sized to them.
+hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, bool small,
+                       hipStream_t st);
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
 // Several bins in one persistent launch.  Bin bin[i] holds count_i =
 // counts[count_stride*bin[i]] units starting at begin_i = range ?

@@ -124,7 +124,7 @@ def test_edge_cases(engine, oracle):
 
 
 def test_record_with_many_haps(engine, plain_engine, oracle):
-    """One record with more haps than the binning keeps in LDS (700 > 512):
+    """One record with more haps than the binning keeps in LDS (700 > 128):
     hap segments, item runs and -inf runs then read hap lengths from device
     memory; some haps are empty or hold a wildcard."""
     import gpupairhmm_amd as G
