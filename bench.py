@@ -192,6 +192,41 @@ def bench_config1_mode(G, batch, mode_name, steps=10, warmup=3):
             "ms_per_step": round(dt / steps * 1e3, 3)}
 
 
+def bench_engine_time(G, batch, calls=5):
+    """The reference's engine time (gpuworker.cuh:562, 966-969): one
+    phmm_score_batch call from host buffers -- staging, H2D, compute and D2H
+    -- per pass over the batch."""
+    with G.Engine(mode=G.MODE_BATCHED) as eng:
+        eng.score_batch(batch)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            eng.score_batch(batch)
+        dt = (time.perf_counter() - t0) / calls
+    return {"gcups": round(batch.cells() / dt / 1e9, 2), "ms_per_call": round(dt * 1e3, 3),
+            "calls": calls, "path": "host buffers: staging + H2D + compute + D2H"}
+
+
+def bench_config0(G, seed=1000, steps=20):
+    """Configuration 0, the reference's CPU-runnable case (1,000 records x 8
+    reads of 100 bp x 4 haps of 120 bp): the batched engine from
+    device-resident inputs, and align_host on every host thread."""
+    from oracle import oracle as O
+    batch = G.HostBatch.synthetic(0, seed=seed)
+    cells = batch.cells()
+    with G.Engine(mode=G.MODE_BATCHED) as eng:
+        db = eng.upload(batch)
+        d = eng.alloc(4 * batch.num_alignments)
+        dt = timed_steps(eng, db, d, steps, 3) / steps
+        eng.free(d)
+        db.free()
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    dc = _time_cpu(batch, O, cores)
+    return {"records": batch.num_groups, "alignments": batch.num_alignments, "cells": cells,
+            "gpu_gcups": round(cells / dt / 1e9, 2), "gpu_ms_per_call": round(dt * 1e3, 3),
+            "cpu_gcups": round(cells / dc / 1e9, 4), "cpu_cores": cores,
+            "cpu_kind": "reference" if O.ref_lib() is not None else "port"}
+
+
 def bench_serving(G, rates, seconds, pool_records, seed):
     """Poisson arrivals of configuration-4 requests (one record each)."""
     pool = [G.HostBatch.synthetic(4, seed=seed + i, num_records=1) for i in range(pool_records)]
@@ -324,6 +359,10 @@ def main():
         other = "plain" if args.mode == "batched" else "batched"
         log(f"configuration 1: {other} path ...")
         result[f"config1_{other}"] = bench_config1_mode(G, batch, other)
+        log("configuration 1: engine time from host buffers ...")
+        result["config1_engine_time"] = bench_engine_time(G, batch)
+        log("configuration 0 ...")
+        result["config0"] = bench_config0(G)
         log("configuration 2: batched vs plain per-record ...")
         result["mixed_batched_vs_plain"] = bench_mixed(G, records=150, seed=1002)
         log("configuration 2: batched, 2000 records ...")
