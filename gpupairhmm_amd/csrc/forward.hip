@@ -86,10 +86,9 @@ __device__ __forceinline__ int wave_min(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Setup-time division: FP32 coefficients use the hardware reciprocal (a few
-// ulp; the recurrence's result is unaffected at its tolerance), FP64 keeps
-// the correctly rounded quotient.
-__device__ __forceinline__ float qdiv(float a, float b) { return __fdividef(a, b); }
+// Setup-time division (once per task and row, about 1% of a task's
+// instructions): the correctly rounded quotient in both precisions.
+__device__ __forceinline__ float qdiv(float a, float b) { return a / b; }
 __device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
 
 __device__ __forceinline__ float ph2pr_of(const float* table, int q)
