@@ -271,9 +271,12 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         bp.align_off[G] = carry;
 }
 
-__device__ __forceinline__ int key_of(int bin, int hl)
+// Sort key of a unit of group g: bin, then descending hap length (or
+// segment columns), then g's part -- the part only spreads the atomics of
+// many groups on one length over kKeyParts counters.
+__device__ __forceinline__ int key_of(int bin, int hl, int g)
 {
-    return bin * kHapBuckets + (kHapBuckets - 1 - hap_bucket(hl));
+    return bin * kKeysPerBin + (kHapBuckets - 1 - hap_bucket(hl)) * kKeyParts + (g & (kKeyParts - 1));
 }
 
 // A group's non-empty read bins (including the empty-read slot NB), built
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             if (k != bp.NB && is_tile_bin(bp, k)) {
                 const ChunkPlan c = chunk_plan(bp, k, gb.cnt[e]);
                 if (c.nfull > 0)
-                    atomicAdd(&bp.key_count[key_of(k, gh.work[sg])], c.nfull);
+                    atomicAdd(&bp.key_count[key_of(k, gh.work[sg], g)], c.nfull);
             }
         }
         // per (hap, bin) item run: the one-wave bins' alignments, and a tile
@@ -524,31 +527,63 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             if (is_tile_bin(bp, k) && !other) {
                 const ChunkPlan c = chunk_plan(bp, k, gb.cnt[e]);
                 if (c.rem > 0)
-                    atomicAdd(&bp.key_count[key_of(item_bin_of(bp, k), hl)], c.rem);
+                    atomicAdd(&bp.key_count[key_of(item_bin_of(bp, k), hl, g)], c.rem);
                 continue;
             }
-            atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl)], gb.cnt[e]);
+            atomicAdd(&bp.key_count[key_of(is_tile_bin(bp, k) ? eb : k, hl, g)], gb.cnt[e]);
         }
         __syncthreads();
     }
 }
 
-// Exclusive scan of each bin's hap-length histogram (one block per bin):
-// key_cursor holds positions relative to the bin's begin, the bin's count
-// goes to bin_range[2*bin+1].
+// Exclusive scan of each bin's key histogram (one block per bin): a thread
+// per hap-length bucket adds the bucket's kKeyParts counters, the block scans
+// the buckets, and each counter's cursor is its bucket's start plus the
+// counters before it in the bucket.  key_cursor holds positions relative to
+// the bin's begin, the bin's count goes to bin_range[2*bin+1].
 __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
 {
+    static_assert(kKeyParts % 4 == 0, "buckets load as int4");
+    static_assert(kAlignParts <= 1024, "one part per thread");
     __shared__ int wsum[16];
     __shared__ int carry;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int bin = blockIdx.x; bin < bp.NW; bin += gridDim.x) {
+        if (bin < bp.NB) {  // fold the bin's alignment-count parts into part 0
+            int v = (int)threadIdx.x < kAlignParts ? bp.align_part[bin * kAlignParts + threadIdx.x] : 0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+                v += __shfl_xor(v, o, 64);
+            if (lane == 0)
+                wsum[wv] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int t = 0;
+                for (int k = 0; k < 16; ++k)
+                    t += wsum[k];
+                bp.align_part[bin * kAlignParts] = t;
+            }
+            __syncthreads();
+        }
         if (threadIdx.x == 0)
             carry = 0;
         __syncthreads();
-        const int base0 = bin * kHapBuckets;
+        const int base0 = bin * kKeysPerBin;
         for (int base = 0; base < kHapBuckets; base += 1024) {
-            const int i = base + threadIdx.x;
-            const int v = i < kHapBuckets ? bp.key_count[base0 + i] : 0;
+            const int bk = base + threadIdx.x;
+            int c[kKeyParts];
+            int v = 0;
+#pragma unroll
+            for (int q = 0; q < kKeyParts / 4; ++q) {
+                int4 x = make_int4(0, 0, 0, 0);
+                if (bk < kHapBuckets)
+                    x = reinterpret_cast<const int4*>(bp.key_count + base0 + bk * kKeyParts)[q];
+                c[4 * q] = x.x;
+                c[4 * q + 1] = x.y;
+                c[4 * q + 2] = x.z;
+                c[4 * q + 3] = x.w;
+                v += x.x + x.y + x.z + x.w;
+            }
             int x = v;
             for (int o = 1; o < 64; o <<= 1) {
                 const int y = __shfl_up(x, o, 64);
@@ -568,8 +603,22 @@ __global__ __launch_bounds__(1024) void k_scan_keys(BinParams bp)
             }
             __syncthreads();
             const int incl = x + wsum[wv] + carry;
-            if (i < kHapBuckets)
-                bp.key_cursor[base0 + i] = incl - v;
+            if (bk < kHapBuckets) {
+                int run = incl - v;
+#pragma unroll
+                for (int q = 0; q < kKeyParts / 4; ++q) {
+                    int4 y;
+                    y.x = run;
+                    run += c[4 * q];
+                    y.y = run;
+                    run += c[4 * q + 1];
+                    y.z = run;
+                    run += c[4 * q + 2];
+                    y.w = run;
+                    run += c[4 * q + 3];
+                    reinterpret_cast<int4*>(bp.key_cursor + base0 + bk * kKeyParts)[q] = y;
+                }
+            }
             __syncthreads();
             if (threadIdx.x == 1023)
                 carry = incl;
@@ -618,13 +667,10 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
             *bp.hmax = m;
     }
     scan_pairs(bp.bin_range, bp.NW, nullptr);
-    for (int k = threadIdx.x; k < bp.NB; k += 64) {  // each bin's alignment count from its parts
-        int v = 0;
-#pragma unroll 16
-        for (int q = 0; q < kAlignParts; ++q)
-            v += bp.align_part[k * kAlignParts + q];
-        tot[k] = v;
-    }
+    // each bin's alignment count (k_scan_keys folded the bin's parts into
+    // part 0)
+    for (int k = threadIdx.x; k < bp.NB; k += 64)
+        tot[k] = bp.align_part[k * kAlignParts];
     __syncthreads();
     scan_pairs(bp.align_range, bp.NB, tot);
 }
@@ -672,7 +718,7 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
             t.aoff = (int32_t)A;
             if (cp.nfull > 0) {
                 const int base = bp.bin_range[2 * k] +
-                                 atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg])], cp.nfull);
+                                 atomicAdd(&bp.key_cursor[key_of(k, gh.work[sg], g)], cp.nfull);
                 t.nc = cp.W;
                 t.he = gh.he[sg];
                 for (int j = 0; j < cp.nfull; ++j) {
@@ -707,11 +753,11 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                             const int kb = item_bin_of(bp, k);
                             base = cp.rem == 0 ? kSkip
                                                : bp.bin_range[2 * kb] +
-                                                     atomicAdd(&bp.key_cursor[key_of(kb, hl)], cp.rem);
+                                                     atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], cp.rem);
                         } else {
                             const int kb = is_tile_bin(bp, k) ? eb : k;
                             base = bp.bin_range[2 * kb] +
-                                   atomicAdd(&bp.key_cursor[key_of(kb, hl)], gb.cnt[e]);
+                                   atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], gb.cnt[e]);
                         }
                     }
                 }
@@ -827,7 +873,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     };
     ClearList cl{};
     cl.p[0] = bp.key_count;
-    cl.n[0] = bp.NW * kHapBuckets;
+    cl.n[0] = bp.NW * kKeysPerBin;
     cl.p[1] = bp.hmax_part;
     cl.n[1] = kHmaxParts;
     cl.p[2] = bp.rescue_counts;

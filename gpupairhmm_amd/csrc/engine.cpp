@@ -216,8 +216,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(gb_start, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
         (rc = ensure(read_perm, sizeof(int32_t) * (size_t)std::max(v.num_reads, 1), "read perm")) ||
         (rc = ensure(align_off, sizeof(int64_t) * (size_t)(G + 1), "align offsets")) ||
-        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NW * kHapBuckets, "keys")) ||
-        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NW * kHapBuckets, "keys")) ||
+        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
+        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
         (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NW, "bin ranges")) ||
         (rc = ensure_zeroed(smallints, 1024, "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||

@@ -65,7 +65,9 @@ __host__ __device__ inline bool tile_rows_supported(int r)
 constexpr int kTaskCounters = 96;   // set launches per call (plain mode: one per bin)
 constexpr int kF64Sets = 8;         // of them, the last ones: FP64 tile sets
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
-constexpr int kAlignParts = 64;     // partial counters per bin (atomic spreading)
+constexpr int kKeyParts = 8;        // counters per (bin, bucket): groups spread over them
+constexpr int kKeysPerBin = kHapBuckets * kKeyParts;
+constexpr int kAlignParts = 256;    // partial counters per bin (atomic spreading)
 constexpr int kHmaxParts = 32;      // partial maxima of the hap pass (atomic spreading)
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
@@ -168,8 +170,8 @@ struct BinParams {
     int32_t* gb_start;         // [G*(NB+1)]
     int32_t* read_perm;        // [nR]
     int64_t* align_off;        // [G+1]
-    int32_t* key_count;        // [NW*kHapBuckets]
-    int32_t* key_cursor;       // [NW*kHapBuckets]
+    int32_t* key_count;        // [NW*kKeysPerBin]
+    int32_t* key_cursor;       // [NW*kKeysPerBin]
     int32_t* bin_range;        // [2*NW]: begin, count (tile bins: super-tasks; others: items)
     int32_t* hmax;             // [1]: the longest hap (k_scan_bins folds hmax_part)
     int32_t* hmax_part;        // [kHmaxParts]: per-block maxima of the hap pass
@@ -179,7 +181,7 @@ struct BinParams {
     int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
     SuperTask* st;             // tile bins' work, in bin_range units
     int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
-    int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin
+    int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin (k_scan_keys sums each bin's into its first)
     uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T
     WorkItem* items;
     const float* ph2pr;
