@@ -659,7 +659,10 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
 {
     __shared__ int tot[kMaxBins];
     {  // the longest hap, from the hap pass's per-block maxima
-        int m = (int)threadIdx.x < kHmaxParts ? bp.hmax_part[threadIdx.x] : 0;
+        int m = 0;
+#pragma unroll
+        for (int q = threadIdx.x; q < kHmaxParts; q += 64)
+            m = max(m, bp.hmax_part[q]);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
             m = max(m, __shfl_xor(m, o, 64));

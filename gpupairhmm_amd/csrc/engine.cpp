@@ -219,7 +219,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         (rc = ensure(key_count, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
         (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
         (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NW, "bin ranges")) ||
-        (rc = ensure_zeroed(smallints, 1024, "counters")) ||
+        (rc = ensure_zeroed(smallints, sizeof(int32_t) * (256 + kHmaxParts), "counters")) ||
         (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
         (rc = ensure(align_range, sizeof(int32_t) * (2 + kAlignParts) * (size_t)NB,
                      "alignment ranges")) ||
@@ -294,8 +294,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
 
     // counters: [0] max hap length, [1] final-list count, [2] error flag,
     // [4 .. 4+NB) per-bin rescue counts, [128 .. 128+kTaskCounters) task
-    // counters of the set launches, [224 .. 224+kHmaxParts) partial maxima
-    static_assert(128 + kTaskCounters <= 224 && 224 + kHmaxParts <= 256, "counter layout");
+    // counters of the set launches, [256 .. 256+kHmaxParts) partial maxima
+    static_assert(4 + kMaxBins <= 128 && 128 + kTaskCounters <= 256, "counter layout");
     int32_t* sm = static_cast<int32_t*>(smallints.p);
     BinParams bp{};
     bp.bd = static_cast<const BatchDesc*>(desc.p);
@@ -312,7 +312,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
     bp.bin_range = static_cast<int32_t*>(bin_range.p);
     bp.hmax = sm + 0;
-    bp.hmax_part = sm + 224;
+    bp.hmax_part = sm + 256;
     bp.final_count = sm + 1;
     bp.error_flag = sm + 2;
     bp.rescue_counts = sm + 4;

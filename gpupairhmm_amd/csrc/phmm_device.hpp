@@ -68,7 +68,7 @@ constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 constexpr int kKeyParts = 8;        // counters per (bin, bucket): groups spread over them
 constexpr int kKeysPerBin = kHapBuckets * kKeyParts;
 constexpr int kAlignParts = 256;    // partial counters per bin (atomic spreading)
-constexpr int kHmaxParts = 32;      // partial maxima of the hap pass (atomic spreading)
+constexpr int kHmaxParts = 512;     // partial maxima of the hap pass (atomic spreading)
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
 __host__ __device__ inline int hap_bucket(int h)
