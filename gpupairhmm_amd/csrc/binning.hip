@@ -122,10 +122,19 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
     // every 16-lane slot runs the same number of iterations, so the row-local
     // reductions below never have a slot missing
     const int iters = (b.nR + nslots - 1) / nslots;
+    const int i0 = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
+    // the next read's length and offset are loaded one read ahead
+    int L_next = i0 < b.nR ? b.read_len[i0] : 0;
+    int off_next = i0 < b.nR ? b.read_off[i0] : 0;
     for (int it = 0; it < iters; ++it) {
-        const int i = (blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4)) + it * nslots;
+        const int i = i0 + it * nslots;
         const bool valid = i < b.nR;
-        const int L = valid ? b.read_len[i] : 0;
+        const int L = valid ? L_next : 0;
+        const int off_i = off_next;
+        if (i + nslots < b.nR) {
+            L_next = b.read_len[i + nslots];
+            off_next = b.read_off[i + nslots];
+        }
         int k = bp.ntiles;
         for (int j = 0; j < bp.ntiles; ++j)
             if (L <= bp.tiles[j]) {
@@ -133,7 +142,7 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
                 break;
             }
         const bool tiled = valid && L > 0 && k < bp.ntiles;
-        const int off = (valid && L > 0) ? b.read_off[i] : 0;
+        const int off = (valid && L > 0) ? off_i : 0;
         const int nd = (valid && L > 0) ? (L >> 2) + 1 : 0;  // dwords of the padded read
         float eps = kEpsGpu;
         if (tiled && bp.semantics == 1)
@@ -175,17 +184,18 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
                 pdelta = up_delta;
                 palpha = up_alpha;
             }
+            // branch-free per row (bitwise logic and selects: no divergent
+            // control flow inside the slot)
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
                 const int r = 4 * d + x;
-                if (r < L) {
-                    exotic = exotic || row_code(w[x]) == kExoticCode;
-                    if (!(al[x] > 0.f && al[x] <= 1.f && dl[x] > 0.f && sg[x] > 0.f))
-                        ok = false;
-                    else if (r > 0 && !(eps * pdelta / (dl[x] * palpha) <= 0.5f))
-                        ok = false;
-                    nl *= (double)al[x];
-                }
+                const bool in = r < L;
+                const bool range_bad = !((al[x] > 0.f) & (al[x] <= 1.f) & (dl[x] > 0.f) & (sg[x] > 0.f));
+                // e_r = eps*delta_{r-1} / (delta_r*alpha_{r-1}) > 1/2
+                const bool er_bad = (r > 0) & !(eps * pdelta <= 0.5f * (dl[x] * palpha));
+                ok = ok & !(in & (range_bad | er_bad));
+                exotic = exotic | (in & (row_code(w[x]) == kExoticCode));
+                nl *= in ? (double)al[x] : 1.0;
                 pdelta = dl[x];
                 palpha = al[x];
             }

@@ -1,7 +1,7 @@
 """The read pass's choice of arithmetic form, checked against a numpy
 restatement of its rule: a read takes the rescaled 5-op form when every row
 has 0 < alpha <= 1 with delta, sigma > 0, every row below the first has
-eps*delta_{r-1} / (delta_r*alpha_{r-1}) <= 1/2 (float arithmetic), and the
+eps*delta_{r-1} <= 1/2 * (delta_r*alpha_{r-1}) (float arithmetic), and the
 product of alpha over the read (double) is at least 1e-30.  Plain mode counts
 the alignments of rescaled-form bins, so the count must match exactly."""
 import numpy as np
@@ -54,7 +54,7 @@ def _fast(P, iq, dq):
         return False
     num = (eps * delta[:-1]).astype(f32)
     den = (delta[1:] * alpha[:-1]).astype(f32)
-    if not np.all((num / den).astype(f32) <= f32(0.5)):
+    if not np.all(num <= (f32(0.5) * den).astype(f32)):
         return False
     return float(np.prod(alpha.astype(np.float64))) >= 1e-30
 
