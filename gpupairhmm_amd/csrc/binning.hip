@@ -58,8 +58,16 @@ __global__ __launch_bounds__(256) void k_hap_pass(BinParams bp)
     const uint32_t* hd = reinterpret_cast<const uint32_t*>(b.hap_data);
     uint32_t* out = reinterpret_cast<uint32_t*>(bp.hapc);
     int m = 0;
-    for (int h = blockIdx.x * (blockDim.x >> 6) + wv; h < b.nH; h += nwaves) {
-        const int hl = b.hap_len[h], d0 = b.hap_off[h] >> 2;
+    const int h0 = blockIdx.x * (blockDim.x >> 6) + wv;
+    // the next hap's length and offset are loaded one hap ahead
+    int hl_next = h0 < b.nH ? b.hap_len[h0] : 0;
+    int off_next = h0 < b.nH ? b.hap_off[h0] : 0;
+    for (int h = h0; h < b.nH; h += nwaves) {
+        const int hl = hl_next, d0 = off_next >> 2;
+        if (h + nwaves < b.nH) {
+            hl_next = b.hap_len[h + nwaves];
+            off_next = b.hap_off[h + nwaves];
+        }
         m = max(m, hl);
         const int nd = (max(hl, 0) >> 2) + 1;  // padded length: hl + 4 - hl % 4
         bool other = false;
@@ -70,7 +78,7 @@ __global__ __launch_bounds__(256) void k_hap_pass(BinParams bp)
             for (int k = 0; k < 4; ++k) {
                 const uint32_t code = (uint32_t)base_code_sem((uint8_t)(w >> (8 * k)), bp.semantics);
                 c |= code << (8 * k);
-                other = other || (4 * d + k < hl && code >= 4u);
+                other = other | ((4 * d + k < hl) & (code >= 4u));
             }
             out[d0 + d] = c;
         }
@@ -237,7 +245,8 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
 }
 
 // Exclusive prefix of R_g*H_g (the reference's resultOffsetsPerBatch,
-// gpuworker.cuh:802-815), one block of 1024 threads.
+// gpuworker.cuh:802-815), one block of 1024 threads, four consecutive
+// groups per thread and pass.
 __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
@@ -248,9 +257,13 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         carry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int base = 0; base < G; base += 1024) {
-        const int g = base + threadIdx.x;
-        int64_t v = g < G ? (int64_t)b.nreads[g] * b.nhaps[g] : 0;
+    for (int base = 0; base < G; base += 4096) {
+        const int g0 = base + 4 * (int)threadIdx.x;
+        int64_t p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            p[u] = g0 + u < G ? (int64_t)b.nreads[g0 + u] * b.nhaps[g0 + u] : 0;
+        const int64_t v = p[0] + p[1] + p[2] + p[3];
         int64_t x = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int64_t y = __shfl_up(x, o, 64);
@@ -270,8 +283,13 @@ __global__ __launch_bounds__(1024) void k_align_offsets(BinParams bp)
         }
         __syncthreads();
         const int64_t incl = x + wsum[wv] + carry;
-        if (g < G)
-            bp.align_off[g] = incl - v;
+        int64_t run = incl - v;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (g0 + u < G)
+                bp.align_off[g0 + u] = run;
+            run += p[u];
+        }
         __syncthreads();
         if (threadIdx.x == 1023)
             carry = incl;
