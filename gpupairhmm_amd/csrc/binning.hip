@@ -42,6 +42,7 @@ __global__ void k_clear_list(ClearList cl)
 // Sequence bytes are padded to multiples of 4 per read/hap (the reference's
 // padding rule) and start 4-byte aligned, so both passes below load and
 // encode 4 bytes per lane: dword loads of each input array.
+
 // One wave per hap: its code bytes (dwords over the padded length), whether
 // it holds a byte other than A/C/G/T within its length (the wildcard --
 // reference GPU semantics: any such byte; CPU semantics: 'N' -- or, under
