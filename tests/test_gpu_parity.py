@@ -100,7 +100,7 @@ def _record(reads, haps, q=30, iq=40, dq=40, gcp=10):
     return "\n".join(lines) + "\n"
 
 
-def test_edge_cases(engine, oracle):
+def test_edge_cases(engine, plain_engine, oracle):
     import gpupairhmm_amd as G
     rng = np.random.default_rng(3)
     rs = lambda n: "".join(rng.choice(list("ACGT"), n)) if n else ""
@@ -119,8 +119,13 @@ def test_edge_cases(engine, oracle):
     got = engine.score_batch(b)
     ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
     assert _max_delta(got, ref) <= TOL
-    st = engine.stats()
-    assert st.multitile_alignments >= 0
+    # the plain engine counts what each bin launched: the last record's four
+    # alignments (reads of 1500 and 1100 > 1023, the largest tile) are
+    # multi-tile, and its scores equal the batched engine's
+    plain_engine.reset_stats()
+    got_plain = plain_engine.score_batch(b)
+    assert np.array_equal(got_plain.view(np.uint32), got.view(np.uint32))
+    assert plain_engine.stats().multitile_alignments >= 4
 
 
 def test_record_with_many_haps(engine, plain_engine, oracle):
