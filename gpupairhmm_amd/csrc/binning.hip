@@ -308,15 +308,16 @@ __device__ __forceinline__ int key_of(int bin, int hl, int g)
     return bin * kKeysPerBin + (kHapBuckets - 1 - hap_bucket(hl)) * kKeyParts + (g & (kKeyParts - 1));
 }
 
-// A group's non-empty read bins (including the empty-read slot NB), built
-// by one wave in LDS: bin id, count and start in the group's read_perm.
+// A group's non-empty read bins (including the empty-read slot NB) in LDS:
+// bin id, count and start in the group's read_perm.  Every wave of the
+// block runs the build (a block of several waves writes the same values).
 struct GroupBins {
     int n;
     int k[kMaxBins + 1], cnt[kMaxBins + 1], st[kMaxBins + 1];
 };
 __device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, GroupBins& gb)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int NB1 = bp.NB + 1;
     if (lane == 0)
         gb.n = 0;
@@ -339,25 +340,25 @@ __device__ __forceinline__ void group_bins_compact(const BinParams& bp, int g, G
     }
 }
 
-// One wave per group: count the group's reads per bin (slot NB: empty
+// The block of a group: count the group's reads per bin (slot NB: empty
 // reads), store the counts, starts and the bin-ordered read permutation for
 // the scatter, and keep the non-empty bins in LDS (as group_bins_compact
 // would read them back).
 __device__ __forceinline__ void group_bins_build(const BinParams& bp, const BatchDesc& b, int g,
                                                  GroupBins& gb, int* cnt, int* pos)
 {
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int NB1 = bp.NB + 1;
-    for (int k = lane; k < NB1; k += 64)
+    for (int k = tid; k < NB1; k += nt)
         cnt[k] = 0;
     __syncthreads();
     const int r0 = b.rprefix[g], nr = b.nreads[g];
-    for (int r = lane; r < nr; r += 64) {
+    for (int r = tid; r < nr; r += nt) {
         const int k = bp.meta[r0 + r].bin;
         atomicAdd(&cnt[k < 0 ? bp.NB : k], 1);
     }
     __syncthreads();
-    if (lane == 0) {
+    if (tid == 0) {
         int s = 0, n = 0;
         for (int k = 0; k < NB1; ++k) {
             const int c = cnt[k];
@@ -375,7 +376,7 @@ __device__ __forceinline__ void group_bins_build(const BinParams& bp, const Batc
         gb.n = n;
     }
     __syncthreads();
-    for (int r = lane; r < nr; r += 64) {
+    for (int r = tid; r < nr; r += nt) {
         const int k = bp.meta[r0 + r].bin;
         const int at = atomicAdd(&pos[k < 0 ? bp.NB : k], 1);
         bp.read_perm[r0 + at] = r0 + r;
@@ -383,7 +384,8 @@ __device__ __forceinline__ void group_bins_build(const BinParams& bp, const Batc
     __syncthreads();
 }
 
-// Per-group hap summary, uniform across the wave: haps with a sequence and
+// Per-group hap summary, uniform across the block (every wave computes it
+// and writes the same values): haps with a sequence and
 // those of them that go to the exact bin (a byte other than A/C/G/T); the others are
 // cut, in order, into segments of at most `budget` columns (at least one hap
 // each) -- the super-tasks' hap ranges and their work (sort key).
@@ -412,7 +414,7 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
                                            GroupHaps& gh)
 {
     int ne = 0, ex = 0, tot = 0;
-    for (int h = threadIdx.x; h < nh; h += 64) {
+    for (int h = (int)(threadIdx.x & 63); h < nh; h += 64) {
         const int hl = b.hap_len[h0 + h];
         if (hl > 0) {
             ++ne;
@@ -435,7 +437,7 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
     // parallel load per chunk instead of a serial chain of global loads)
     int n = 0, cur = 0, start = 0;
     for (int c0 = 0; c0 < nh; c0 += 64) {
-        const int h = c0 + (int)threadIdx.x;
+        const int h = c0 + (int)(threadIdx.x & 63);
         int cols = 0;
         if (h < nh) {
             const int hl = b.hap_len[h0 + h];
@@ -444,9 +446,9 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
             if (h < kMaxSegs)
                 gh.info[h] = (hl > 0 ? hl : 0) | (flag ? (int)0x80000000u : 0);
         }
-        gh.cols[threadIdx.x] = cols;
+        gh.cols[threadIdx.x & 63] = cols;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if ((threadIdx.x & 63) == 0) {
             for (int i = 0; i < min(64, nh - c0); ++i) {
                 const int hl = gh.cols[i];
                 if (hl == 0)
@@ -465,7 +467,7 @@ __device__ __forceinline__ void group_haps(const BinParams& bp, const BatchDesc&
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if ((threadIdx.x & 63) == 0) {
         gh.nonempty = ne;
         gh.other = ex;
         if (cur > 0) {
@@ -498,12 +500,33 @@ __device__ __forceinline__ ChunkPlan chunk_plan(const BinParams& bp, int k, int 
 // The item bin of tile bin k (work-bin ids past the multi-tile and exact bins).
 __device__ __forceinline__ int item_bin_of(const BinParams& bp, int k) { return bp.NB + k; }
 
-// Per group (one wave): its reads per bin (group_bins_build), the per-bin
+// Records with many (hap, bin) runs -- haps x min(reads, 64) above
+// kBigGroupRuns -- are counted and scattered by a block of four waves
+// (k_count_keys_t<4>, k_scatter_t<4>), the others by one wave each: one wave
+// handling a large record's runs in turn is latency-bound.
+constexpr int kBigGroupRuns = 512;
+__device__ __forceinline__ bool big_group(const BatchDesc& b, int g)
+{
+    return (int64_t)b.nhaps[g] * min(b.nreads[g], 64) > kBigGroupRuns;
+}
+
+// Orders LDS traffic between the lanes of one wave (the waves of a block
+// run different numbers of hap chunks, so no block barrier applies there).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per group (NW = 1: one wave, the small groups; NW = 4: a block of four
+// waves, the large ones): its reads per bin (group_bins_build), the per-bin
 // alignment counts and the sort-key histogram.  A tile bin's full chunks get
 // one key (bin, segment columns) per hap segment, counted in super-tasks;
 // item bins, the multi-tile and the exact bins get one key (bin, hap length)
 // per (hap, bin) run, counted in alignments.
-__global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_count_keys_t(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     __shared__ GroupBins gb;
@@ -511,6 +534,8 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
     __shared__ int cnt[kMaxBins + 1], pos[kMaxBins + 1];
     const int eb = 2 * bp.ntiles + 1;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        if (big_group(b, g) != (NW > 1))  // uniform over the block
+            continue;
         group_bins_build(bp, b, g, gb, cnt, pos);
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         group_haps(bp, b, h0, nh, gh);
@@ -519,7 +544,7 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
         // per bin (every group adds to its bins: one counter per bin would
         // serialize the atomics of all groups)
         const int part = g & (kAlignParts - 1);
-        for (int e = threadIdx.x; e < n; e += 64) {
+        for (int e = threadIdx.x; e < n; e += 64 * NW) {
             const int k = gb.k[e], c = gb.cnt[e];
             if (k == bp.NB)
                 continue;
@@ -534,7 +559,7 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
             }
         }
         // per (tile bin entry, hap segment): its super-tasks' key
-        for (int x = threadIdx.x; x < n * gh.nseg; x += 64) {
+        for (int x = threadIdx.x; x < n * gh.nseg; x += 64 * NW) {
             const int e = x / gh.nseg, sg = x - e * gh.nseg;
             const int k = gb.k[e];
             if (k != bp.NB && is_tile_bin(bp, k)) {
@@ -545,7 +570,7 @@ __global__ __launch_bounds__(64) void k_count_keys(BinParams bp)
         }
         // per (hap, bin) item run: the one-wave bins' alignments, and a tile
         // bin's left-over reads against each of its haps
-        for (int x = threadIdx.x; x < nh * n; x += 64) {
+        for (int x = threadIdx.x; x < nh * n; x += 64 * NW) {
             const int h = x / n, e = x - h * n;
             const int k = gb.k[e];
             int hl;
@@ -707,36 +732,164 @@ __global__ __launch_bounds__(64) void k_scan_bins(BinParams bp)
     scan_pairs(bp.align_range, bp.NB, tot);
 }
 
-// Per group (one wave): write the work lists.  Tile bins: the group's reads
-// of the bin in full chunks of 64/lanes, one super-task per chunk and hap
-// segment; the reads left over go to the bin's item bin.  Item bins, the
-// multi-tile and the exact bins: one item per alignment, reserved per (hap,
-// bin) run -- one lane per pair, haps in chunks that fit the LDS table --
-// and written with lanes over the reads of a run.  Alignments with an empty read
-// or hap get -inf directly (align_host returns log10(0) - log10(c) for them).
-__global__ __launch_bounds__(64) void k_scatter(BinParams bp)
+// One group's item runs of haps [hc, hc + hcn) (at most kRuns (hap, bin
+// entry) pairs), written by one wave: lanes reserve the runs -- tile-bin
+// pairs are covered by super-tasks -- and list the others compactly; then
+// the wave writes each listed run, lanes over the run's reads.  Alignments
+// with an empty read or hap get -inf directly (align_host returns
+// log10(0) - log10(c) for them).
+template <int kRuns, bool kWaveSync>
+__device__ __forceinline__ void scatter_hap_chunk(const BinParams& bp, const BatchDesc& b,
+                                                  const GroupBins& gb, const GroupHaps& gh, int g,
+                                                  int r0, int h0, int nh, int64_t A, int hc,
+                                                  int hcn, int* run_x, int* run_base, int* w_off,
+                                                  int* w_st, int* w_h, int* w_base)
+{
+    constexpr int kSkip = -2;  // run covered by a super-task
+    const int lane = threadIdx.x & 63;
+    const int eb = 2 * bp.ntiles + 1;
+    const int n = gb.n;
+    auto sync = [] {
+        if (kWaveSync)
+            wave_sync();
+        else
+            __syncthreads();
+    };
+    int nrun = 0;  // uniform
+    for (int x0 = 0; x0 < hcn * n; x0 += 64) {  // same trip count on every lane
+        const int x = x0 + lane;
+        int base = kSkip;
+        if (x < hcn * n) {
+            const int h = hc + x / n, e = x % n;
+            const int k = gb.k[e];
+            int hl;
+            bool other;
+            hap_info(bp, b, gh, h0, h, hl, other);
+            base = -1;  // -inf
+            if (hl > 0 && k != bp.NB) {
+                if (is_tile_bin(bp, k) && !other) {
+                    // super-tasks cover the full chunks; left-over reads
+                    // become items of the bin's item bin
+                    const ChunkPlan cp = chunk_plan(bp, k, gb.cnt[e]);
+                    const int kb = item_bin_of(bp, k);
+                    base = cp.rem == 0 ? kSkip
+                                       : bp.bin_range[2 * kb] +
+                                             atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], cp.rem);
+                } else {
+                    const int kb = is_tile_bin(bp, k) ? eb : k;
+                    base = bp.bin_range[2 * kb] + atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], gb.cnt[e]);
+                }
+            }
+        }
+        const bool keep = base != kSkip;
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int pos = nrun + __popcll(m & ((1ull << lane) - 1ull));
+            run_x[pos] = x;
+            run_base[pos] = base;
+        }
+        nrun += __popcll(m);
+    }
+    sync();
+    // write the listed runs 64 at a time, lanes over the runs' (run, read)
+    // pairs: many runs are a read or two (left-over reads), and every pair
+    // costs a dependent load of read_perm
+    for (int q0 = 0; q0 < nrun; q0 += 64) {  // same trip count on every lane
+        const int q = q0 + lane;
+        int cnt = 0, st = 0, h = 0, base = 0;
+        if (q < nrun) {
+            const int x = run_x[q];
+            const int e = x % n;
+            h = hc + x / n;
+            base = run_base[q];
+            cnt = gb.cnt[e];
+            st = gb.st[e];
+            if (base >= 0 && is_tile_bin(bp, gb.k[e])) {
+                int hl;
+                bool other;
+                hap_info(bp, b, gh, h0, h, hl, other);
+                if (!other) {  // an item-bin run: the left-over reads only
+                    const ChunkPlan cp = chunk_plan(bp, gb.k[e], cnt);
+                    st += cp.nfull * cp.W;
+                    cnt = cp.rem;
+                }
+            }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o)
+                incl += y;
+        }
+        const int total = __shfl(incl, 63, 64);
+        w_off[lane] = incl - cnt;
+        w_st[lane] = st;
+        w_h[lane] = h;
+        w_base[lane] = base;
+        sync();
+        const int last = min(64, nrun - q0) - 1;
+        for (int f = lane; f < total; f += 64) {
+            int lo = 0, hi = last;  // the last run whose offset is <= f
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (w_off[mid] <= f)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int j = f - w_off[lo];
+            const int rd = bp.read_perm[r0 + w_st[lo] + j];
+            const int64_t out = A + (int64_t)(rd - r0) * nh + w_h[lo];
+            if (w_base[lo] < 0) {
+                b.scores[out] = -INFINITY;  // empty read or empty hap
+            } else {
+                WorkItem it;
+                it.read = rd;
+                it.hap = h0 + w_h[lo];
+                it.out = (int32_t)out;
+                it.pad = 0;
+                bp.items[w_base[lo] + j] = it;
+            }
+        }
+        sync();
+    }
+    sync();
+}
+
+// Write the work lists, per group.  Tile bins: the group's reads of the bin
+// in full chunks of 64/lanes, one super-task per chunk and hap segment; the
+// reads left over go to the bin's item bin.  Item bins, the multi-tile and
+// the exact bins: one item per alignment, reserved per (hap, bin) run (haps
+// in chunks that fit the LDS table).  NW = 1: one wave per group, the
+// small groups (big_group).  NW = 4: a block of four waves per group, the
+// others; its waves share the super-task pairs and take the hap
+// chunks in turn.  Both launches stride over all groups and skip the other
+// kind.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_scatter_t(BinParams bp)
 {
     const BatchDesc& b = *bp.bd;
     constexpr int kRuns = 256;
-    constexpr int kSkip = -2;  // run covered by a super-task
     __shared__ GroupBins gb;
     __shared__ GroupHaps gh;
-    __shared__ int run_x[kRuns], run_base[kRuns];
-    __shared__ int w_off[64], w_st[64], w_h[64], w_base[64];
-    const int lane = threadIdx.x;
-    const int eb = 2 * bp.ntiles + 1;
+    __shared__ int run_x[NW][kRuns], run_base[NW][kRuns];
+    __shared__ int w_off[NW][64], w_st[NW][64], w_h[NW][64], w_base[NW][64];
+    const int wv = threadIdx.x >> 6;
     for (int g = blockIdx.x; g < b.G; g += gridDim.x) {
+        if (big_group(b, g) != (NW > 1))  // uniform over the block
+            continue;
         group_bins_compact(bp, g, gb);
         const int r0 = b.rprefix[g];
         const int h0 = b.hprefix[g], nh = b.nhaps[g];
         const int64_t A = bp.align_off[g];
         group_haps(bp, b, h0, nh, gh);
         const int n = gb.n;
-        // super-tasks of the tile bins: read chunks x hap segments, one lane
-        // per (bin entry, segment) pair: it reserves the pair's list slots
-        // and writes its tasks
+        // super-tasks of the tile bins: read chunks x hap segments, one
+        // thread per (bin entry, segment) pair: it reserves the pair's list
+        // slots and writes its tasks
         const int npairs = n * gh.nseg;
-        for (int x = lane; x < npairs; x += 64) {
+        for (int x = threadIdx.x; x < npairs; x += 64 * NW) {
             const int e = x / gh.nseg, sg = x - e * gh.nseg;
             const int k = gb.k[e];
             if (k == bp.NB || !is_tile_bin(bp, k))
@@ -759,115 +912,12 @@ __global__ __launch_bounds__(64) void k_scatter(BinParams bp)
                 }
             }
         }
-        // item runs and -inf: lanes reserve the runs of (hap, bin entry)
-        // pairs -- tile-bin pairs are covered by super-tasks -- and list the
-        // others compactly; then the wave writes each listed run, lanes over
-        // the run's reads
         const int hchunk = n > 0 ? max(1, kRuns / n) : nh;
-        for (int hc = 0; hc < nh; hc += hchunk) {
-            const int hcn = min(hchunk, nh - hc);
-            int nrun = 0;  // uniform
-            for (int x0 = 0; x0 < hcn * n; x0 += 64) {  // same trip count on every lane
-                const int x = x0 + lane;
-                int base = kSkip;
-                if (x < hcn * n) {
-                    const int h = hc + x / n, e = x % n;
-                    const int k = gb.k[e];
-                    int hl;
-                    bool other;
-                    hap_info(bp, b, gh, h0, h, hl, other);
-                    base = -1;  // -inf
-                    if (hl > 0 && k != bp.NB) {
-                        if (is_tile_bin(bp, k) && !other) {
-                            // super-tasks cover the full chunks; left-over
-                            // reads become items of the bin's item bin
-                            const ChunkPlan cp = chunk_plan(bp, k, gb.cnt[e]);
-                            const int kb = item_bin_of(bp, k);
-                            base = cp.rem == 0 ? kSkip
-                                               : bp.bin_range[2 * kb] +
-                                                     atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], cp.rem);
-                        } else {
-                            const int kb = is_tile_bin(bp, k) ? eb : k;
-                            base = bp.bin_range[2 * kb] +
-                                   atomicAdd(&bp.key_cursor[key_of(kb, hl, g)], gb.cnt[e]);
-                        }
-                    }
-                }
-                const bool keep = base != kSkip;
-                const unsigned long long m = __ballot(keep);
-                if (keep) {
-                    const int pos = nrun + __popcll(m & ((1ull << lane) - 1ull));
-                    run_x[pos] = x;
-                    run_base[pos] = base;
-                }
-                nrun += __popcll(m);
-            }
-            __syncthreads();
-            // write the listed runs 64 at a time, lanes over the runs' (run,
-            // read) pairs: many runs are a read or two (left-over reads), and
-            // every pair costs a dependent load of read_perm
-            for (int q0 = 0; q0 < nrun; q0 += 64) {  // same trip count on every lane
-                const int q = q0 + lane;
-                int cnt = 0, st = 0, h = 0, base = 0;
-                if (q < nrun) {
-                    const int x = run_x[q];
-                    const int e = x % n;
-                    h = hc + x / n;
-                    base = run_base[q];
-                    cnt = gb.cnt[e];
-                    st = gb.st[e];
-                    if (base >= 0 && is_tile_bin(bp, gb.k[e])) {
-                        int hl;
-                        bool other;
-                        hap_info(bp, b, gh, h0, h, hl, other);
-                        if (!other) {  // an item-bin run: the left-over reads only
-                            const ChunkPlan cp = chunk_plan(bp, gb.k[e], cnt);
-                            st += cp.nfull * cp.W;
-                            cnt = cp.rem;
-                        }
-                    }
-                }
-                int incl = cnt;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int y = __shfl_up(incl, o, 64);
-                    if (lane >= o)
-                        incl += y;
-                }
-                const int total = __shfl(incl, 63, 64);
-                w_off[lane] = incl - cnt;
-                w_st[lane] = st;
-                w_h[lane] = h;
-                w_base[lane] = base;
-                __syncthreads();
-                const int last = min(64, nrun - q0) - 1;
-                for (int f = lane; f < total; f += 64) {
-                    int lo = 0, hi = last;  // the last run whose offset is <= f
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (w_off[mid] <= f)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    const int j = f - w_off[lo];
-                    const int rd = bp.read_perm[r0 + w_st[lo] + j];
-                    const int64_t out = A + (int64_t)(rd - r0) * nh + w_h[lo];
-                    if (w_base[lo] < 0) {
-                        b.scores[out] = -INFINITY;  // empty read or empty hap
-                    } else {
-                        WorkItem it;
-                        it.read = rd;
-                        it.hap = h0 + w_h[lo];
-                        it.out = (int32_t)out;
-                        it.pad = 0;
-                        bp.items[w_base[lo] + j] = it;
-                    }
-                }
-                __syncthreads();
-            }
-            __syncthreads();
-        }
+        for (int hc = wv * hchunk; hc < nh; hc += NW * hchunk)
+            scatter_hap_chunk<kRuns, (NW > 1)>(bp, b, gb, gh, g, r0, h0, nh, A, hc,
+                                               min(hchunk, nh - hc), run_x[wv], run_base[wv],
+                                               w_off[wv], w_st[wv], w_h[wv], w_base[wv]);
+        __syncthreads();  // the next group rebuilds gb and gh
     }
 }
 
@@ -924,12 +974,15 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
         hipLaunchKernelGGL(k_read_meta, dim3(grid(hd.nR, tpb / 16, 4096)), dim3(tpb), 0, st, bp);
     hipLaunchKernelGGL(k_align_offsets, dim3(1), dim3(1024), 0, st, bp);
     if (fixed_grids || hd.G > 0) {
-        hipLaunchKernelGGL(k_count_keys, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
+        hipLaunchKernelGGL(k_count_keys_t<1>, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
+        hipLaunchKernelGGL(k_count_keys_t<4>, dim3(grid(hd.G, 1, 2048)), dim3(256), 0, st, bp);
     }
     hipLaunchKernelGGL(k_scan_keys, dim3(bp.NW), dim3(1024), 0, st, bp);
     hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(64), 0, st, bp);
-    if (fixed_grids || hd.G > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
+    if (fixed_grids || hd.G > 0) {
+        hipLaunchKernelGGL(k_scatter_t<1>, dim3(grid(hd.G, 1, 8192)), dim3(64), 0, st, bp);
+        hipLaunchKernelGGL(k_scatter_t<4>, dim3(grid(hd.G, 1, 2048)), dim3(256), 0, st, bp);
+    }
     return hipGetLastError();
 }
 
