@@ -591,6 +591,12 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
                 best_cost = cost;
             }
         }
+        if (best < 0) {  // no tile holds the class: one wave per alignment
+            wave.bin[wave.n] = b;
+            wave.lanes[wave.n] = 64;
+            ++wave.n;
+            continue;
+        }
         BinSet& x = set[best][fast];
         x.bin[x.n] = b;
         x.lanes[x.n] = (tile + kNr64[best] - 1) / kNr64[best];
