@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_align_host.so")
 
 F64, F32CELLS = 0, 1
-SEM_GPU, SEM_CPU = 0, 1
+SEM_GPU, SEM_CPU, SEM_GCP = 0, 1, 2
 
 FLT_MAX = float(np.finfo(np.float32).max)
 CONSTANT = FLT_MAX / 16.0
@@ -55,6 +55,9 @@ def lib():
         _lib.oracle_align.restype = ctypes.c_double
         _lib.oracle_align.argtypes = [u8p, ctypes.c_int, u8p, ctypes.c_int, u8p, u8p, u8p,
                                       ctypes.c_uint8, ctypes.c_int, ctypes.c_int]
+        _lib.oracle_align_gcp.restype = ctypes.c_double
+        _lib.oracle_align_gcp.argtypes = [u8p, ctypes.c_int, u8p, ctypes.c_int, u8p, u8p, u8p,
+                                          u8p, ctypes.c_int, ctypes.c_int]
         _lib.oracle_ph2pr.argtypes = [ctypes.POINTER(ctypes.c_float)]
         batch_args = [ctypes.c_int, ip, ip, ip, ip, u8p, u8p, u8p, u8p, u8p, u8p, ip, ip, ip, ip]
         _lib.oracle_score_batch.restype = ctypes.c_int
@@ -111,6 +114,20 @@ def align(hap, read, base_q, ins_q, del_q, gcp0: int = 10,
     return lib().oracle_align(_ptr(h, ctypes.c_uint8), len(h), _ptr(r, ctypes.c_uint8), len(r),
                               _ptr(bq, ctypes.c_uint8), _ptr(iq, ctypes.c_uint8),
                               _ptr(dq, ctypes.c_uint8), int(gcp0), precision, semantics)
+
+
+def align_gcp(hap, read, base_q, ins_q, del_q, gcp_q, precision: int = F64,
+              semantics: int = SEM_GCP) -> float:
+    """One alignment with the read's gcp qualities (one per base), the form
+    that serves the per-position gcp semantics."""
+    h, r = _u8(hap), _u8(read)
+    bq, iq, dq, gq = _u8(base_q), _u8(ins_q), _u8(del_q), _u8(gcp_q)
+    if len(gq) < len(r):
+        raise ValueError("one gcp quality per read base")
+    return lib().oracle_align_gcp(_ptr(h, ctypes.c_uint8), len(h), _ptr(r, ctypes.c_uint8),
+                                  len(r), _ptr(bq, ctypes.c_uint8), _ptr(iq, ctypes.c_uint8),
+                                  _ptr(dq, ctypes.c_uint8), _ptr(gq, ctypes.c_uint8), precision,
+                                  semantics)
 
 
 def ref_align(hap, read, base_q, ins_q, del_q, gcp0: int = 10) -> float:
@@ -190,10 +207,15 @@ def score_subset(b, which, precision: int = F64, semantics: int = SEM_GPU,
 # Brute force: explicit enumeration of every state path (tiny inputs only).
 # ---------------------------------------------------------------------------
 
-def brute_force(hap: bytes, read: bytes, base_q, ins_q, del_q, eps: float = 0.1,
-                beta: float = 0.9) -> float:
+def brute_force(hap: bytes, read: bytes, base_q, ins_q, del_q, eps=0.1, beta=0.9,
+                literal_n: bool = False) -> float:
     """Sum, over every path through the pair HMM, of the product of its
     transition and emission probabilities -- no dynamic programming.
+
+    eps and beta are numbers, or sequences of one value per read row (row r
+    at index r-1: the per-position gcp model).  literal_n: only the byte 'N'
+    is a wildcard (the reference CPU rule), else every byte other than
+    A/C/G/T.
 
     The model (the one the reference's forward recurrence computes,
     gpuworker.cuh:1043-1094): a path starts in the deletion state above hap
@@ -203,12 +225,17 @@ def brute_force(hap: bytes, read: bytes, base_q, ins_q, del_q, eps: float = 0.1,
       M(r,c) -> M(r+1,c+1)  alpha_{r+1} * emit(r+1,c+1)
       M(r,c) -> I(r+1,c)    delta_{r+1}
       M(r,c) -> D(r,c+1)    sigma_r
-      I(r,c) -> M(r+1,c+1)  beta * emit ;  I(r,c) -> I(r+1,c)  eps
-      D(r,c) -> M(r+1,c+1)  beta * emit ;  D(r,c) -> D(r,c+1)  eps
-    A path ends in M or I on the last read row.  Returns log10 of the sum
-    (the reference's score with its scaling constant cancelled).
+      I(r,c) -> M(r+1,c+1)  beta_{r+1} * emit ;  I(r,c) -> I(r+1,c)  eps_{r+1}
+      D(r,c) -> M(r+1,c+1)  beta_{r+1} * emit ;  D(r,c) -> D(r,c+1)  eps_r
+    (the start enters row 1 with beta_1).  A path ends in M or I on the last
+    read row.  Returns log10 of the sum (the reference's score with its
+    scaling constant cancelled).
     """
     R, H = len(read), len(hap)
+    ep = list(eps) if hasattr(eps, "__len__") else [eps] * R
+    bt = list(beta) if hasattr(beta, "__len__") else [beta] * R
+    if len(ep) < R or len(bt) < R:
+        raise ValueError("one eps and beta per read row")
     P = [10.0 ** (-q / 10.0) for q in range(128)]
     p = [P[min(q, 127)] for q in base_q]
     dl = [P[min(q, 127)] for q in ins_q]
@@ -219,12 +246,17 @@ def brute_force(hap: bytes, read: bytes, base_q, ins_q, del_q, eps: float = 0.1,
         return {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}.get(b, 4)
 
     def emit(r, c):  # 1-based
-        x, h = code(read[r - 1]), code(hap[c - 1])
-        return 1.0 - p[r - 1] if (x == h or x == 4 or h == 4) else p[r - 1] / 3.0
+        x, h = read[r - 1], hap[c - 1]
+        if literal_n:
+            m = x == h or x == ord("N") or h == ord("N")
+        else:
+            x, h = code(x), code(h)
+            m = x == h or x == 4 or h == 4
+        return 1.0 - p[r - 1] if m else p[r - 1] / 3.0
 
     total = 0.0
     # explicit depth-first enumeration of paths: (state, r, c, weight)
-    stack = [("M", 1, c0 + 1, beta * emit(1, c0 + 1) / H) for c0 in range(H)]
+    stack = [("M", 1, c0 + 1, bt[0] * emit(1, c0 + 1) / H) for c0 in range(H)]
     while stack:
         s, r, c, w = stack.pop()
         if r == R and s in ("M", "I"):
@@ -238,12 +270,12 @@ def brute_force(hap: bytes, read: bytes, base_q, ins_q, del_q, eps: float = 0.1,
                 stack.append(("D", r, c + 1, w * sg[r - 1]))
         elif s == "I":
             if r < R and c < H:
-                stack.append(("M", r + 1, c + 1, w * beta * emit(r + 1, c + 1)))
+                stack.append(("M", r + 1, c + 1, w * bt[r] * emit(r + 1, c + 1)))
             if r < R:
-                stack.append(("I", r + 1, c, w * eps))
+                stack.append(("I", r + 1, c, w * ep[r]))
         else:  # D
             if r < R and c < H:
-                stack.append(("M", r + 1, c + 1, w * beta * emit(r + 1, c + 1)))
+                stack.append(("M", r + 1, c + 1, w * bt[r] * emit(r + 1, c + 1)))
             if c < H:
-                stack.append(("D", r, c + 1, w * eps))
+                stack.append(("D", r, c + 1, w * ep[r - 1]))
     return math.log10(total) if total > 0 else float("-inf")

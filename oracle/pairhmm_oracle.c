@@ -36,7 +36,7 @@ static int base_code(uint8_t b)
 
 static int is_match(uint8_t x, uint8_t h, int semantics)
 {
-    if (semantics == ORACLE_SEM_CPU) /* gpuworker.cuh:1063 */
+    if (semantics != ORACLE_SEM_GPU) /* gpuworker.cuh:1063 */
         return x == h || x == 'N' || h == 'N';
     /* GPU: both sides encoded, code 4 matches everything */
     int cx = base_code(x), ch = base_code(h);
@@ -97,19 +97,23 @@ static inline xd xd_add(xd a, xd b)
     return a;
 }
 
+/* Gap continuation eps and gap-to-match beta of row r (1-based):
+ *   SEM_GPU: fixed 0.1f / 0.9f (pairhmm_kernels.cuh:1195-1196);
+ *   SEM_CPU: from the read's first gcp quality for every row
+ *            (gpuworker.cuh:1009, 1048-1049);
+ *   SEM_GCP: from the row's own gcp quality gq[r-1], indexed like the row's
+ *            insertion and deletion qualities (gpuworker.cuh:1059-1061). */
+static inline uint8_t gcp_of_row(const uint8_t* gq, int r, int semantics)
+{
+    return semantics == ORACLE_SEM_GCP ? gq[r - 1] : gq[0];
+}
+
 static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
                         const uint8_t* bq, const uint8_t* iq, const uint8_t* dq,
-                        uint8_t gcp0, int semantics, const float* P)
+                        const uint8_t* gq, int semantics, const float* P)
 {
     const double constant = (double)FLT_MAX / 16.0;
-    double eps, beta;
-    if (semantics == ORACLE_SEM_CPU) {
-        eps = (double)P[qclamp(gcp0)];
-        beta = 1.0 - eps;
-    } else {
-        eps = (double)0.1f;
-        beta = (double)0.9f;
-    }
+    double eps = (double)0.1f, beta = (double)0.9f;
     xd* buf = (xd*)malloc(sizeof(xd) * 6 * (size_t)(H + 1));
     xd *Mp = buf, *Ip = buf + (H + 1), *Dp = buf + 2 * (H + 1);
     xd *Mc = buf + 3 * (H + 1), *Ic = buf + 4 * (H + 1), *Dc = buf + 5 * (H + 1);
@@ -124,6 +128,10 @@ static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
         const double delta = P[qclamp(iq[r - 1])];
         const double sigma = P[qclamp(dq[r - 1])];
         const double alpha = 1.0 - (delta + sigma);
+        if (semantics != ORACLE_SEM_GPU) {
+            eps = (double)P[qclamp(gcp_of_row(gq, r, semantics))];
+            beta = 1.0 - eps;
+        }
         Mc[0] = Ic[0] = Dc[0] = zero;
         for (int c = 1; c <= H; ++c) {
             const double lambda = is_match(read[r - 1], hap[c - 1], semantics) ? 1.0 - p : p / 3.0;
@@ -149,7 +157,7 @@ static double align_f64(const uint8_t* hap, int H, const uint8_t* read, int R,
 /* Float cells, double result: the rounding of align_host, step for step. */
 static double align_f32cells(const uint8_t* hap, int H, const uint8_t* read, int R,
                              const uint8_t* bq, const uint8_t* iq, const uint8_t* dq,
-                             uint8_t gcp0, int semantics, const float* P)
+                             const uint8_t* gq, int semantics, const float* P)
 {
     double result = 0;
     const double constant = (double)FLT_MAX / 16;
@@ -161,15 +169,13 @@ static double align_f32cells(const uint8_t* hap, int H, const uint8_t* read, int
         M[col] = I[col] = 0;
         D[col] = (float)(constant / H);
     }
-    float beta, eps;
-    if (semantics == ORACLE_SEM_CPU) {
-        beta = (float)(1.0 - P[qclamp(gcp0)]);
-        eps = P[qclamp(gcp0)];
-    } else {
-        eps = (float)0.1;
-        beta = (float)0.9;
-    }
+    float beta = (float)0.9, eps = (float)0.1;
     for (int row = 1; row <= R; row++) {
+        if (semantics != ORACLE_SEM_GPU) {
+            const uint8_t g = gcp_of_row(gq, row, semantics);
+            beta = (float)(1.0 - P[qclamp(g)]);
+            eps = P[qclamp(g)];
+        }
         target_row = row & 1;
         source_row = !target_row;
         M[target_row * W] = I[target_row * W] = 0;
@@ -204,17 +210,28 @@ static double align_f32cells(const uint8_t* hap, int H, const uint8_t* read, int
     return result;
 }
 
-double oracle_align(const uint8_t* hap, int hap_len, const uint8_t* read, int read_len,
-                    const uint8_t* base_q, const uint8_t* ins_q, const uint8_t* del_q,
-                    uint8_t gcp0, int precision, int semantics)
+double oracle_align_gcp(const uint8_t* hap, int hap_len, const uint8_t* read, int read_len,
+                        const uint8_t* base_q, const uint8_t* ins_q, const uint8_t* del_q,
+                        const uint8_t* gcp_q, int precision, int semantics)
 {
     float P[128];
     oracle_ph2pr(P);
     if (hap_len <= 0 || read_len <= 0)
         return -INFINITY;
     if (precision == ORACLE_F32CELLS)
-        return align_f32cells(hap, hap_len, read, read_len, base_q, ins_q, del_q, gcp0, semantics, P);
-    return align_f64(hap, hap_len, read, read_len, base_q, ins_q, del_q, gcp0, semantics, P);
+        return align_f32cells(hap, hap_len, read, read_len, base_q, ins_q, del_q, gcp_q, semantics,
+                              P);
+    return align_f64(hap, hap_len, read, read_len, base_q, ins_q, del_q, gcp_q, semantics, P);
+}
+
+double oracle_align(const uint8_t* hap, int hap_len, const uint8_t* read, int read_len,
+                    const uint8_t* base_q, const uint8_t* ins_q, const uint8_t* del_q,
+                    uint8_t gcp0, int precision, int semantics)
+{
+    if (semantics == ORACLE_SEM_GCP)  /* needs every row's gcp */
+        return NAN;
+    return oracle_align_gcp(hap, hap_len, read, read_len, base_q, ins_q, del_q, &gcp0, precision,
+                            semantics);
 }
 
 /* Map a global read-major output index to (group, read, hap) exactly as
@@ -247,9 +264,9 @@ static double score_one(int64_t a, int G, const int* nH,
     const int roff = ro[read], hoff = ho[hap];
     if (precision == ORACLE_F32CELLS)
         return align_f32cells(hap_data + hoff, HL, read_data + roff, L, bq + roff, iq + roff,
-                              dq + roff, gq[roff], semantics, P);
+                              dq + roff, gq + roff, semantics, P);
     return align_f64(hap_data + hoff, HL, read_data + roff, L, bq + roff, iq + roff, dq + roff,
-                     gq[roff], semantics, P);
+                     gq + roff, semantics, P);
 }
 
 static int64_t* alignment_prefix(int G, const int* nR, const int* nH)

@@ -20,7 +20,14 @@
  *                       than A/C/G/T is the N wildcard (utility_kernels.cuh:16-20);
  *   - ORACLE_SEM_CPU  : what align_host computes: eps/beta from the read's
  *                       first gcp quality (gpuworker.cuh:1009,1048-1049) and
- *                       only a literal 'N' is a wildcard (gpuworker.cuh:1063).
+ *                       only a literal 'N' is a wildcard (gpuworker.cuh:1063);
+ *   - ORACLE_SEM_GCP  : align_host with every row's own gap continuation:
+ *                       row r (1-based) takes eps_r = ph2pr[gcp[r-1]] and
+ *                       beta_r = 1 - eps_r where align_host takes eps and
+ *                       beta, indexed like the row's insertion and deletion
+ *                       qualities (gpuworker.cuh:1059-1061); literal 'N'
+ *                       wildcard.  Equal to ORACLE_SEM_CPU, bit for bit, when
+ *                       a read's gcp qualities are all equal.
  * Both semantics agree for uppercase ACGTN input with gcp '+'.
  *
  * Pinning (see tests/test_oracle.py): the README known-answer record
@@ -43,7 +50,7 @@ extern "C" {
 #endif
 
 enum { ORACLE_F64 = 0, ORACLE_F32CELLS = 1 };
-enum { ORACLE_SEM_GPU = 0, ORACLE_SEM_CPU = 1 };
+enum { ORACLE_SEM_GPU = 0, ORACLE_SEM_CPU = 1, ORACLE_SEM_GCP = 2 };
 
 /* ph2pr[q] = (float)10^(-q/10), q = 0..127  (gpuworker.cuh:473-486). */
 void oracle_ph2pr(float out[128]);
@@ -56,6 +63,14 @@ double oracle_align(const uint8_t* hap, int hap_len,
                     const uint8_t* base_q, const uint8_t* ins_q,
                     const uint8_t* del_q, uint8_t gcp0,
                     int precision, int semantics);
+
+/* The same with the read's gcp qualities (gcp_q[0..read_len-1]); the only
+ * form that serves ORACLE_SEM_GCP (oracle_align returns NaN for it). */
+double oracle_align_gcp(const uint8_t* hap, int hap_len,
+                        const uint8_t* read, int read_len,
+                        const uint8_t* base_q, const uint8_t* ins_q,
+                        const uint8_t* del_q, const uint8_t* gcp_q,
+                        int precision, int semantics);
 
 /* A whole batch in the reference's SoA layout (common.cuh:109-201), scored
  * like processBatchCPU: out[off_g + r*H_g + h].  Returns 0 on success.
