@@ -30,7 +30,7 @@ import numpy as np
 __all__ = [
     "PairHMMError", "HostBatch", "DeviceBatch", "Engine", "Server", "load_library",
     "library_path", "format_scores", "build", "MODE_PLAIN", "MODE_BATCHED", "SEM_GPU",
-    "SEM_CPU", "EXPORTED_SYMBOLS",
+    "SEM_CPU", "SEM_GCP", "EXPORTED_SYMBOLS",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -38,7 +38,7 @@ LIB_NAME = "libgpupairhmm.so"
 
 OK, EINVAL, ENOMEM, EDEVICE, EIO, EPARSE, EBUSY, EAGAIN, ERANGE = range(9)
 MODE_PLAIN, MODE_BATCHED = 0, 1
-SEM_GPU, SEM_CPU = 0, 1
+SEM_GPU, SEM_CPU, SEM_GCP = 0, 1, 2
 
 # Every symbol include/gpupairhmm.h declares.
 EXPORTED_SYMBOLS = (
@@ -52,7 +52,7 @@ EXPORTED_SYMBOLS = (
     "phmm_synth_generate", "phmm_host_batch_to_text", "phmm_format_scores", "phmm_free_text",
     "phmm_batch_cells", "phmm_server_options_default", "phmm_server_create",
     "phmm_server_destroy", "phmm_server_submit", "phmm_server_poll", "phmm_server_stats_get",
-    "phmm_compute_likelihoods",
+    "phmm_compute_likelihoods", "phmm_host_alloc_pinned", "phmm_host_free_pinned",
 )
 
 
@@ -126,7 +126,8 @@ class ServerOptions(ctypes.Structure):
     _fields_ = [
         ("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
         ("max_batch_requests", ctypes.c_int32),
-        ("max_batch_alignments", ctypes.c_int64), ("reserved", ctypes.c_int32 * 6),
+        ("max_batch_alignments", ctypes.c_int64), ("max_batch_bytes", ctypes.c_int64),
+        ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -135,7 +136,9 @@ class ServerStats(ctypes.Structure):
         ("submitted", ctypes.c_int64), ("refused", ctypes.c_int64), ("completed", ctypes.c_int64),
         ("launches", ctypes.c_int64), ("inflight", ctypes.c_int64), ("queued", ctypes.c_int64),
         ("max_queue_depth", ctypes.c_int64), ("bytes_in_use", ctypes.c_int64),
-        ("cells", ctypes.c_int64), ("reserved", ctypes.c_int64 * 8),
+        ("cells", ctypes.c_int64), ("scratch_bytes", ctypes.c_int64),
+        ("arena_bytes", ctypes.c_int64), ("failed", ctypes.c_int64),
+        ("max_concurrent", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4),
     ]
 
 
@@ -194,6 +197,8 @@ def load_library():
         "phmm_device_batch_free": (None, [vp]),
         "phmm_device_alloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "phmm_device_free": (ctypes.c_int, [vp, vp]),
+        "phmm_host_alloc_pinned": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "phmm_host_free_pinned": (ctypes.c_int, [vp, vp]),
         "phmm_memcpy_d2h": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
         "phmm_memcpy_h2d": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
         "phmm_host_batch_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t,
@@ -474,6 +479,18 @@ class EngineStats:
     graph_captures: int = 0
 
 
+class _PinnedBlocks:
+    """Page-locked host blocks of a batch, returned with it."""
+
+    def __init__(self, engine, blocks):
+        self.engine, self.blocks = engine, blocks
+
+    def __del__(self):
+        if _LIB is not None and getattr(self.engine, "_h", None) is not None:
+            for p in self.blocks:
+                _LIB.phmm_host_free_pinned(self.engine._h, ctypes.c_void_p(p))
+
+
 class DeviceBatch:
     """A batch copied to engine-owned device memory."""
 
@@ -607,6 +624,25 @@ class Engine:
                                                  ctypes.c_void_p(d_ptr), out.nbytes))
         return out
 
+    def pinned_copy(self, batch: HostBatch) -> HostBatch:
+        """A copy of `batch` whose arrays live in page-locked host memory
+        (freed with the returned batch): phmm_score_batch copies them to the
+        device without staging."""
+        lib = load_library()
+        arrays, blocks = {}, []
+        for name in _VIEW_ARRAYS:
+            a = batch.array(name)
+            p = ctypes.c_void_p()
+            self._chk(lib.phmm_host_alloc_pinned(self._h, max(a.nbytes, 1), ctypes.byref(p)))
+            blocks.append(p.value)
+            buf = (ctypes.c_uint8 * max(a.nbytes, 1)).from_address(p.value)
+            dst = np.frombuffer(buf, dtype=a.dtype, count=a.size)
+            dst[:] = a
+            arrays[name] = dst
+        out = HostBatch.from_arrays(**arrays)
+        out._pinned = _PinnedBlocks(self, blocks)
+        return out
+
     def score_resident(self, dbatch: DeviceBatch) -> np.ndarray:
         """Score a device batch and copy the scores back (convenience)."""
         n = max(dbatch.num_alignments, 1)
@@ -645,7 +681,8 @@ class Server:
 
     def __init__(self, engine: Engine, hbm_budget_bytes: int = 64 << 30,
                  max_inflight: int = 1 << 20, max_batch_requests: int = 4096,
-                 max_batch_alignments: int | None = None):
+                 max_batch_alignments: int | None = None, max_batch_bytes: int | None = None,
+                 streams: int | None = None):
         lib = load_library()
         o = ServerOptions()
         lib.phmm_server_options_default(ctypes.byref(o))
@@ -654,11 +691,15 @@ class Server:
         o.max_batch_requests = int(max_batch_requests)
         if max_batch_alignments is not None:
             o.max_batch_alignments = int(max_batch_alignments)
+        if max_batch_bytes is not None:
+            o.max_batch_bytes = int(max_batch_bytes)
+        if streams is not None:
+            o.streams = int(streams)
         h = ctypes.c_void_p()
         _check(lib.phmm_server_create(engine._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         self.engine = engine
-        self._keep = {}
+        self._buf = None
 
     def submit(self, request: HostBatch, tag: int) -> bool:
         """True when admitted, False when the budget refused it."""
@@ -668,19 +709,22 @@ class Server:
         _check(rc)
         return True
 
-    def poll(self, timeout_ms: int = 0, capacity: int = 1 << 22):
-        """(tag, scores, latency_seconds) of a finished request, or None."""
+    def poll(self, timeout_ms: int = 0, capacity: int = 1 << 16):
+        """(tag, scores, latency_seconds) of a finished request, or None.
+        A request whose launch failed raises PairHMMError."""
         lib = load_library()
-        buf = np.empty(capacity, dtype=np.float32)
+        if self._buf is None or len(self._buf) < capacity:
+            self._buf = np.empty(capacity, dtype=np.float32)
+        buf = self._buf
         tag = ctypes.c_uint64()
         n = ctypes.c_int64()
         lat = ctypes.c_double()
         rc = lib.phmm_server_poll(self._h, int(timeout_ms), ctypes.byref(tag),
                                   buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                  capacity, ctypes.byref(n), ctypes.byref(lat))
+                                  len(buf), ctypes.byref(n), ctypes.byref(lat))
         if rc == EAGAIN:
             return None
-        if rc == ERANGE:
+        if rc == ERANGE and n.value > len(buf):
             return self.poll(timeout_ms, int(n.value))
         _check(rc)
         return int(tag.value), buf[: n.value].copy(), float(lat.value)

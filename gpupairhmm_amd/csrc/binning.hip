@@ -21,8 +21,8 @@ namespace phmm {
 // Several counter arrays zeroed by one launch (a fill kernel rather than
 // memsets, so the sequence can be captured and replayed as a graph).
 struct ClearList {
-    int32_t* p[6];
-    int32_t n[6];
+    int32_t* p[8];
+    int32_t n[8];
     int32_t count;
 };
 __global__ void k_clear_list(ClearList cl)
@@ -110,8 +110,9 @@ __global__ __launch_bounds__(256) void k_hap_pass(BinParams bp)
 // row whose max_read_len covers the read (its tile has room for one more
 // row); reads beyond the last row go to the multi-tile bin 2*ntiles.  Bin 2k
 // uses the rescaled (FAST) arithmetic when the read's coefficients keep it in
-// range, else bin 2k+1 (SAFE).  Reference CPU semantics: a read holding a
-// byte other than A/C/G/T/N goes to the exact bin 2*ntiles+1.  The slot
+// range, else bin 2k+1 (SAFE); under per-position gcp every row's eps
+// enters the check.  Reference CPU and per-position gcp semantics: a read
+// holding a byte other than A/C/G/T/N goes to the exact bin 2*ntiles+1.  The slot
 // combines its lanes' verdicts and the product of alpha with row-local
 // reductions.
 __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
@@ -125,6 +126,8 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
     const uint32_t* q0 = reinterpret_cast<const uint32_t*>(b.bq);
     const uint32_t* q1 = reinterpret_cast<const uint32_t*>(b.iq);
     const uint32_t* q2 = reinterpret_cast<const uint32_t*>(b.dq);
+    const uint32_t* q3 = reinterpret_cast<const uint32_t*>(b.gcp);
+    const bool row_gcp = bp.semantics == 2;  // per-position gcp: eps of every row
     uint4* rows4 = reinterpret_cast<uint4*>(bp.rows);
     const int sl = threadIdx.x & 15;
     const int nslots = gridDim.x * (blockDim.x >> 4);
@@ -156,6 +159,8 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
         float eps = kEpsGpu;
         if (tiled && bp.semantics == 1)
             eps = ph2pr[min((int)b.gcp[off], 127)];
+        else if (row_gcp)
+            eps = 0.f;  // the rows' own eps are checked below
         bool ok = true, exotic = false;
         double nl = 1.0;
         float pdelta = 0.f, palpha = 1.f;  // the row above this lane's first row
@@ -163,9 +168,12 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
         for (int base = 0; base < nd; base += 16) {
             const int d = base + sl;
             uint32_t w[4] = {0u, 0u, 0u, 0u};
+            uint32_t gw = 0u;  // gcp bytes of the four rows (per-position gcp only)
             if (d < nd) {
                 const int q = (off >> 2) + d;
                 const uint32_t r = rd[q], a = q0[q], c = q1[q], e = q2[q];
+                if (row_gcp)
+                    gw = q3[q];
 #pragma unroll
                 for (int x = 0; x < 4; ++x) {
                     const uint32_t code = (uint32_t)base_code_sem((uint8_t)(r >> (8 * x)), bp.semantics);
@@ -199,9 +207,12 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
             for (int x = 0; x < 4; ++x) {
                 const int r = 4 * d + x;
                 const bool in = r < L;
-                const bool range_bad = !((al[x] > 0.f) & (al[x] <= 1.f) & (dl[x] > 0.f) & (sg[x] > 0.f));
-                // e_r = eps*delta_{r-1} / (delta_r*alpha_{r-1}) > 1/2
-                const bool er_bad = (r > 0) & !(eps * pdelta <= 0.5f * (dl[x] * palpha));
+                // the row's gap continuation (per-position gcp: its own, at most 1/2)
+                const float er = row_gcp ? ph2pr[min((gw >> (8 * x)) & 0xffu, 127u)] : eps;
+                const bool range_bad = !((al[x] > 0.f) & (al[x] <= 1.f) & (dl[x] > 0.f) &
+                                         (sg[x] > 0.f) & (er <= 0.5f));
+                // e_r = eps_r*delta_{r-1} / (delta_r*alpha_{r-1}) > 1/2
+                const bool er_bad = (r > 0) & !(er * pdelta <= 0.5f * (dl[x] * palpha));
                 ok = ok & !(in & (range_bad | er_bad));
                 exotic = exotic | (in & (row_code(w[x]) == kExoticCode));
                 nl *= in ? (double)al[x] : 1.0;
@@ -232,7 +243,7 @@ __global__ __launch_bounds__(256) void k_read_meta(BinParams bp)
             m.bin = -1;
         } else if (k == bp.ntiles) {
             m.bin = 2 * bp.ntiles;
-        } else if (bp.semantics == 1 && exv) {
+        } else if (bp.semantics != 0 && exv) {
             m.bin = 2 * bp.ntiles + 1;
         } else {
             bool fast = okv && bp.allow_fast != 0 && eps <= 0.5f;
@@ -927,8 +938,114 @@ __global__ void k_add_count(int64_t* total, const int32_t* counts, int n)
         int64_t s = 0;
         for (int i = 0; i < n; ++i)
             s += counts[i];
-        *total += s;
+        // calls on several streams may add at once
+        atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)s);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Rescue lists ordered by hap length.  The FP32 pass appends an alignment
+// to its bin's rescue list in completion order, so the alignments the FP64
+// tile kernels take 64/G at a time have unrelated hap lengths and a wave
+// sweeps the longest of them while its other groups idle.  A counting sort
+// per bin (descending length buckets: 8 columns wide below 1024, 64 above)
+// gives every FP64 task alignments of about equal length, longest first.
+// Within a bucket the order is arbitrary: every alignment's arithmetic
+// depends on its own read and hap only.
+
+__device__ __forceinline__ int rescue_key(int hl)
+{
+    int b = hl < 1024 ? (hl >> 3) : 128 + ((hl - 1024) >> 6);
+    b = b < kRescueBuckets - 1 ? b : kRescueBuckets - 1;
+    return kRescueBuckets - 1 - b;  // longest first
+}
+
+// Bucket counts per bin (block-local counts in LDS, one global add per
+// non-empty bucket and block).
+__global__ __launch_bounds__(256) void k_rescue_hist(RescueSort rs)
+{
+    __shared__ int cnt[kRescueBuckets];
+    const BatchDesc& bd = *rs.bd;
+    for (int b = 0; b < rs.nbins; ++b) {
+        const int n = rs.counts[b];
+        if (n == 0)  // uniform: n is read by every thread
+            continue;
+        const int begin = rs.range[2 * b];
+        cnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            atomicAdd(&cnt[rescue_key(bd.hap_len[rs.src[begin + i].hap])], 1);
+        __syncthreads();
+        if (cnt[threadIdx.x] > 0)
+            atomicAdd(&rs.hist[b * kRescueBuckets + threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+// Bucket starts per bin: the bin's list start plus the exclusive prefix of
+// its bucket counts (one block of 256 threads per bin).
+__global__ __launch_bounds__(256) void k_rescue_scan(RescueSort rs)
+{
+    __shared__ int part[kRescueBuckets];
+    const int b = blockIdx.x;
+    if (b >= rs.nbins)
+        return;
+    const int v = rs.hist[b * kRescueBuckets + threadIdx.x];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kRescueBuckets; o <<= 1) {
+        const int y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    rs.cursor[b * kRescueBuckets + threadIdx.x] = rs.range[2 * b] + part[threadIdx.x] - v;
+}
+
+// Scatter into bucket order: positions within a block from LDS counters,
+// one global reservation per non-empty bucket and block pass.
+__global__ __launch_bounds__(256) void k_rescue_scatter(RescueSort rs)
+{
+    __shared__ int cnt[kRescueBuckets];
+    __shared__ int base[kRescueBuckets];
+    const BatchDesc& bd = *rs.bd;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (int b = 0; b < rs.nbins; ++b) {
+        const int n = rs.counts[b];
+        const int begin = rs.range[2 * b];
+        // same trip count on every thread of the block (barriers inside)
+        for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+            const int i = i0 + (int)threadIdx.x;
+            WorkItem it{};
+            int key = 0, local = 0;
+            if (i < n) {
+                it = rs.src[begin + i];
+                key = rescue_key(bd.hap_len[it.hap]);
+                local = atomicAdd(&cnt[key], 1);
+            }
+            __syncthreads();
+            const int c = cnt[threadIdx.x];
+            if (c > 0) {
+                base[threadIdx.x] = atomicAdd(&rs.cursor[b * kRescueBuckets + threadIdx.x], c);
+                cnt[threadIdx.x] = 0;
+            }
+            __syncthreads();
+            if (i < n)
+                rs.dst[base[key] + local] = it;
+        }
+    }
+}
+
+hipError_t run_rescue_sort(const RescueSort& rs, bool fixed_grids, bool small, hipStream_t st)
+{
+    if (rs.nbins <= 0 || rs.nbins > kMaxBins)
+        return hipErrorInvalidValue;
+    const int grid = fixed_grids ? (small ? 128 : 1024) : 1024;
+    hipLaunchKernelGGL(k_rescue_hist, dim3(grid), dim3(256), 0, st, rs);
+    hipLaunchKernelGGL(k_rescue_scan, dim3(rs.nbins), dim3(256), 0, st, rs);
+    hipLaunchKernelGGL(k_rescue_scatter, dim3(grid), dim3(256), 0, st, rs);
+    return hipGetLastError();
 }
 
 // *total += sum of counts[0..n)
@@ -966,7 +1083,9 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     cl.n[4] = kTaskCounters;
     cl.p[5] = bp.align_part;
     cl.n[5] = kAlignParts * bp.NB;
-    cl.count = 6;
+    cl.p[6] = bp.rescue_hist;
+    cl.n[6] = kRescueBuckets * (bp.NB - 2);  // the tile bins' rescue-list buckets
+    cl.count = 7;
     hipLaunchKernelGGL(k_clear_list, dim3(fixed_grids && small ? 128 : 512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.nH > 0)  // one wave per hap
         hipLaunchKernelGGL(k_hap_pass, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);

@@ -3,7 +3,7 @@
 // Same flags and semantics: --inputfile, --outputfile, --fileBatchsize,
 // --transferBatchsize (<= 0: unlimited), --verbose, --queue_depth,
 // --checkResults (accepted, as in the reference it has no effect).  Extra
-// flags: --mode plain|batched, --semantics gpu|cpu, --no-rescue.
+// flags: --mode plain|batched, --semantics gpu|cpu|gcp, --no-rescue.
 //
 // Three threads joined by bounded queues, like the reference pipeline
 // (main.cu:7-27): parser -> engine -> writer.  Scores are written as the
@@ -116,8 +116,10 @@ int main(int argc, char** argv)
                 o.semantics = PHMM_SEM_GPU;
             else if (m == "cpu")
                 o.semantics = PHMM_SEM_CPU;
+            else if (m == "gcp")
+                o.semantics = PHMM_SEM_GCP;
             else
-                die("--semantics must be gpu or cpu");
+                die("--semantics must be gpu, cpu or gcp");
         } else if (a == "--no-rescue")
             o.rescue = false;
     }

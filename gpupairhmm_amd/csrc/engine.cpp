@@ -36,21 +36,28 @@ int fail(int code, const std::string& msg)
 }
 const char* last_error() { return g_last_error.c_str(); }
 
-// Reads and clears the device error flag the multi-tile and rescue kernels
-// raise when the spill arena cannot hold one alignment.
+// Reads and clears the device error flags (one per call context) the
+// multi-tile and rescue kernels raise when the spill arena cannot hold one
+// alignment.
 int Engine::take_error_flag()
 {
-    if (!smallints.p)
-        return PHMM_OK;
-    int32_t* d_flag = static_cast<int32_t*>(smallints.p) + 2;
     int32_t flag = 0;
-    int rc;
-    if ((rc = check(hipMemcpy(&flag, d_flag, sizeof flag, hipMemcpyDeviceToHost), "error flag")))
-        return rc;
+    for (CallCtx& c : ctxs) {
+        if (!c.smallints.p)
+            continue;
+        int32_t* d_flag = static_cast<int32_t*>(c.smallints.p) + 2;
+        int32_t f = 0;
+        int rc;
+        if ((rc = check(hipMemcpy(&f, d_flag, sizeof f, hipMemcpyDeviceToHost), "error flag")))
+            return rc;
+        if (f) {
+            const int32_t zero = 0;
+            hipMemcpy(d_flag, &zero, sizeof zero, hipMemcpyHostToDevice);
+            flag |= f;
+        }
+    }
     if (!flag)
         return PHMM_OK;
-    const int32_t zero = 0;
-    hipMemcpy(d_flag, &zero, sizeof zero, hipMemcpyHostToDevice);
     return set_error(PHMM_ERANGE, "multi-tile/rescue spill arena too small for a batch (flag " +
                                       std::to_string(flag) + ")");
 }
@@ -62,30 +69,14 @@ Engine::~Engine()
         hipStreamSynchronize(stream);
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
-    for (DevBuf* b : {&rows, &hapc, &meta, &gb_count, &gb_start, &read_perm, &align_off, &key_count,
-                      &key_cursor, &bin_range, &align_range, &smallints, &items, &supertasks,
-                      &rescue_items, &final_items, &spill,
-                      &hapflag})
-        if (b->p)
-            hipFree(b->p);
     for (auto& s : slots)
         free_slot(s);
-    drop_graphs();
-    if (desc.p)
-        hipFree(desc.p);
-    if (h_desc)
-        hipHostFree(h_desc);
-    for (auto& ev : desc_ev)
-        if (ev)
-            hipEventDestroy(ev);
-    if (call_done)
-        hipEventDestroy(call_done);
+    for (CallCtx& c : ctxs)
+        free_ctx(c);
     if (cap_stream)
         hipStreamDestroy(cap_stream);
     if (d_rescued_total)
         hipFree(d_rescued_total);
-    if (h_bin_range)
-        hipHostFree(h_bin_range);
     if (h_small)
         hipHostFree(h_small);
     if (d_ph2pr)
@@ -101,7 +92,36 @@ Engine::~Engine()
         hipStreamDestroy(copy_stream);
 }
 
-int Engine::ensure(DevBuf& b, size_t bytes, const char* what)
+void Engine::free_ctx(CallCtx& c)
+{
+    drop_graphs(c);
+    for (DevBuf& b : c.sbuf)
+        if (b.p)
+            hipFree(b.p);
+    for (DevBuf* b : {&c.smallints, &c.spill, &c.desc})
+        if (b->p)
+            hipFree(b->p);
+    if (c.h_desc)
+        hipHostFree(c.h_desc);
+    if (c.h_bin_range)
+        hipHostFree(c.h_bin_range);
+    for (auto& ev : c.desc_ev)
+        if (ev)
+            hipEventDestroy(ev);
+    if (c.call_done)
+        hipEventDestroy(c.call_done);
+    c = CallCtx{};
+}
+
+int Engine::ensure(CallCtx& c, DevBuf& b, size_t bytes, const char* what)
+{
+    if (bytes <= b.cap && b.p)
+        return PHMM_OK;
+    ++c.generation;
+    return grow(b, bytes, what);
+}
+
+int Engine::grow(DevBuf& b, size_t bytes, const char* what)
 {
     if (bytes <= b.cap && b.p)
         return PHMM_OK;
@@ -113,7 +133,6 @@ int Engine::ensure(DevBuf& b, size_t bytes, const char* what)
         b.p = nullptr;
         b.cap = 0;
     }
-    ++generation;
     nb = (nb + 255) & ~(size_t)255;
     if (hipMalloc(&b.p, nb) != hipSuccess) {
         hipGetLastError();
@@ -125,14 +144,91 @@ int Engine::ensure(DevBuf& b, size_t bytes, const char* what)
     return PHMM_OK;
 }
 
-int Engine::ensure_zeroed(DevBuf& b, size_t bytes, const char* what)
+int Engine::init_ctx(int k)
 {
-    if (bytes <= b.cap && b.p)
+    if (k < 0 || k >= kMaxCtx)
+        return set_error(PHMM_EINVAL, "call context out of range");
+    CallCtx& c = ctxs[k];
+    if (c.ready)
         return PHMM_OK;
-    int rc = ensure(b, bytes, what);
-    if (rc)
+    int rc;
+    if ((rc = ensure(c, c.smallints, sizeof(int32_t) * (256 + kHmaxParts), "counters")) ||
+        (rc = check(hipMemset(c.smallints.p, 0, c.smallints.cap), "memset")) ||
+        (rc = ensure(c, c.desc, sizeof(BatchDesc), "batch descriptor")) ||
+        (rc = check(hipHostMalloc(reinterpret_cast<void**>(&c.h_desc),
+                                  sizeof(BatchDesc) * kDescRing, hipHostMallocDefault),
+                    "pinned descriptors")) ||
+        (rc = check(hipHostMalloc(reinterpret_cast<void**>(&c.h_bin_range),
+                                  2 * sizeof(int32_t) * (kMaxWorkBins + kMaxBins),
+                                  hipHostMallocDefault),
+                    "pinned")))
         return rc;
-    return check(hipMemset(b.p, 0, b.cap), "memset");
+    for (int i = 0; i < kDescRing; ++i)
+        if ((rc = check(hipEventCreateWithFlags(&c.desc_ev[i], hipEventDisableTiming), "event")))
+            return rc;
+    if ((rc = check(hipEventCreateWithFlags(&c.call_done, hipEventDisableTiming), "event")))
+        return rc;
+    c.ready = true;
+    return PHMM_OK;
+}
+
+void Engine::bins_for(const phmm_batch_view& v, int* NB, int* NW) const
+{
+    // the table a call of v uses (see score_device)
+    const bool small = roweps() || (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 &&
+                                    v.num_alignments <= opt.small_batch);
+    const int ntiles = (int)(small ? tiles_small : tiles).size();
+    *NB = 2 * ntiles + 2;
+    *NW = *NB + 2 * ntiles;
+}
+
+size_t Engine::ctx_bytes(int k) const
+{
+    const CallCtx& c = ctxs[k];
+    size_t n = c.spill.cap + c.smallints.cap + c.desc.cap;
+    for (const DevBuf& b : c.sbuf)
+        n += b.cap;
+    return n;
+}
+
+int Engine::reserve_ctx(int k, const phmm_batch_view& v)
+{
+    int rc;
+    if ((rc = init_ctx(k)))
+        return rc;
+    CallCtx& c = ctxs[k];
+    int NB, NW;
+    bins_for(v, &NB, &NW);
+    size_t need[kNumScratch];
+    scratch_sizes(v, NB, NW, need);
+    for (int i = 0; i < kNumScratch; ++i)
+        if ((rc = ensure(c, c.sbuf[i], need[i], "scratch")))
+            return rc;
+    return ensure(c, c.spill, c.spill_bytes, "spill");
+}
+
+void Engine::scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[kNumScratch])
+{
+    const size_t G = (size_t)v.num_groups, nA = (size_t)std::max<int64_t>(v.num_alignments, 1);
+    out[kRows] = sizeof(uint32_t) * (size_t)std::max<int64_t>(v.read_bytes, 1);
+    out[kHapc] = (size_t)std::max<int64_t>(v.hap_bytes, 4) + 16;
+    out[kMeta] = sizeof(ReadMeta) * (size_t)std::max(v.num_reads, 1);
+    out[kGbCount] = sizeof(int32_t) * G * (size_t)(NB + 1);
+    out[kGbStart] = out[kGbCount];
+    out[kReadPerm] = sizeof(int32_t) * (size_t)std::max(v.num_reads, 1);
+    out[kAlignOff] = sizeof(int64_t) * (G + 1);
+    out[kKeyCount] = sizeof(int32_t) * (size_t)NW * kKeysPerBin;
+    out[kKeyCursor] = out[kKeyCount];
+    out[kBinRange] = sizeof(int32_t) * 2 * (size_t)NW;
+    out[kItems] = sizeof(WorkItem) * nA;
+    out[kAlignRange] = sizeof(int32_t) * (2 + kAlignParts) * (size_t)NB;
+    // each super-task covers at least one (read, hap) alignment and no two
+    // cover the same one: at most num_alignments of them
+    out[kSuperTasks] = sizeof(SuperTask) * nA;
+    out[kRescueItems] = sizeof(WorkItem) * nA;
+    out[kFinalItems] = sizeof(WorkItem) * nA;
+    out[kHapFlag] = (size_t)std::max(v.num_haps, 1);
+    out[kRescueKeys] = sizeof(int32_t) * 2 * (size_t)kMaxBins * kRescueBuckets;
 }
 
 int Engine::ensure_pin(PinBuf& b, size_t bytes, const char* what)
@@ -172,9 +268,22 @@ int Engine::finish_timing()
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// True when p lies in page-locked host memory (hipHostMalloc'd), which a
+// copy engine reads directly.
+static bool host_pinned(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 // The device part of a batch.  `v` holds device pointers (counts on the
 // host) and must be self-contained (reads_prefix[0] == haps_prefix[0] == 0).
-int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st)
+int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st, char* arena,
+                         size_t arena_bytes, int ctx)
 {
     if (v.num_groups < 0 || v.num_reads < 0 || v.num_haps < 0 || v.num_alignments < 0)
         return set_error(PHMM_EINVAL, "score_device: negative counts");
@@ -193,13 +302,16 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     auto misaligned = [](const void* q) { return ((uintptr_t)q & 3) != 0; };
     if ((v.read_bytes & 3) || (v.hap_bytes & 3) || misaligned(v.read_data) ||
         misaligned(v.base_quals) || misaligned(v.ins_quals) || misaligned(v.del_quals) ||
-        misaligned(v.hap_data))
+        misaligned(v.hap_data) || (opt.semantics == PHMM_SEM_GCP && misaligned(v.gcp_quals)))
         return set_error(PHMM_EINVAL,
                          "score_device: sequence arrays must be 4-byte aligned with padded lengths "
                          "(byte counts multiples of 4)");
     // batched calls of small batches use the one-template table (plain mode
-    // keeps the per-class launches of the tuned table)
-    act = (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 && v.num_alignments <= opt.small_batch)
+    // keeps the per-class launches of the tuned table); per-position gcp
+    // always does: its tile kernels (a gap continuation register per row)
+    // exist for 16 rows per lane
+    act = (roweps() || (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 &&
+                        v.num_alignments <= opt.small_batch))
               ? &tiles_small
               : &tiles;
     const std::vector<phmm_tile_config>& tiles = *act;
@@ -209,44 +321,45 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     const int G = v.num_groups;
     const int64_t nA = v.num_alignments;
     int rc;
-    if ((rc = ensure(rows, sizeof(uint32_t) * (size_t)std::max<int64_t>(v.read_bytes, 1), "rows")) ||
-        (rc = ensure(hapc, (size_t)std::max<int64_t>(v.hap_bytes, 4) + 16, "hap codes")) ||
-        (rc = ensure(meta, sizeof(ReadMeta) * (size_t)std::max(v.num_reads, 1), "read meta")) ||
-        (rc = ensure(gb_count, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
-        (rc = ensure(gb_start, sizeof(int32_t) * (size_t)G * (NB + 1), "group bins")) ||
-        (rc = ensure(read_perm, sizeof(int32_t) * (size_t)std::max(v.num_reads, 1), "read perm")) ||
-        (rc = ensure(align_off, sizeof(int64_t) * (size_t)(G + 1), "align offsets")) ||
-        (rc = ensure(key_count, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
-        (rc = ensure(key_cursor, sizeof(int32_t) * (size_t)NW * kKeysPerBin, "keys")) ||
-        (rc = ensure(bin_range, sizeof(int32_t) * 2 * (size_t)NW, "bin ranges")) ||
-        (rc = ensure_zeroed(smallints, sizeof(int32_t) * (256 + kHmaxParts), "counters")) ||
-        (rc = ensure(items, sizeof(WorkItem) * (size_t)nA, "work items")) ||
-        (rc = ensure(align_range, sizeof(int32_t) * (2 + kAlignParts) * (size_t)NB,
-                     "alignment ranges")) ||
-        // each super-task covers at least one (read, hap) alignment and no
-        // two cover the same one: at most num_alignments of them
-        (rc = ensure(supertasks, sizeof(SuperTask) * (size_t)std::max<int64_t>(nA, 1), "super-tasks")) ||
-        (rc = ensure(rescue_items, sizeof(WorkItem) * (size_t)nA, "rescue list")) ||
-        (rc = ensure(final_items, sizeof(WorkItem) * (size_t)nA, "final list")) ||
-        (rc = ensure(desc, sizeof(BatchDesc), "batch descriptor")) ||
-        (rc = ensure(hapflag, (size_t)std::max(v.num_haps, 1), "hap flags")))
+    if ((rc = init_ctx(ctx)))
         return rc;
-    const size_t spill_bytes = (size_t)256 << 20;
-    if ((rc = ensure(spill, spill_bytes, "spill")))
-        return rc;
-    if (!h_desc) {
-        if ((rc = check(hipHostMalloc(reinterpret_cast<void**>(&h_desc),
-                                      sizeof(BatchDesc) * kDescRing, hipHostMallocDefault),
-                        "pinned descriptors")))
-            return rc;
-        for (int k = 0; k < kDescRing; ++k)
-            if ((rc = check(hipEventCreateWithFlags(&desc_ev[k], hipEventDisableTiming), "event")))
+    CallCtx& c = ctxs[ctx];
+    cur = &c;
+    static const char* const kScratchName[kNumScratch] = {
+        "rows", "hap codes", "read meta", "group bins", "group bins", "read perm",
+        "align offsets", "keys", "keys", "bin ranges", "work items", "alignment ranges",
+        "super-tasks", "rescue list", "final list", "hap flags", "rescue keys"};
+    size_t need[kNumScratch];
+    scratch_sizes(v, NB, NW, need);
+    void* sp[kNumScratch];
+    if (arena) {
+        // the allocation-free form: every buffer at a 256-byte boundary of
+        // the arena, the spill arena takes the rest
+        size_t o = 0;
+        for (int i = 0; i < kNumScratch; ++i) {
+            sp[i] = arena + o;
+            o = align256(o + need[i]);
+        }
+        if (o + kMinArenaSpill > arena_bytes)
+            return set_error(PHMM_ERANGE,
+                             "score_device: the batch needs " + std::to_string(o + kMinArenaSpill) +
+                                 " bytes of device scratch; the arena holds " +
+                                 std::to_string(arena_bytes));
+        c.call_spill = arena + o;
+        c.call_spill_bytes = arena_bytes - o;
+    } else {
+        for (int i = 0; i < kNumScratch; ++i) {
+            if ((rc = ensure(c, c.sbuf[i], need[i], kScratchName[i])))
                 return rc;
-        if ((rc = check(hipEventCreateWithFlags(&call_done, hipEventDisableTiming), "event")))
+            sp[i] = c.sbuf[i].p;
+        }
+        if ((rc = ensure(c, c.spill, c.spill_bytes, "spill")))
             return rc;
+        c.call_spill = c.spill.p;
+        c.call_spill_bytes = c.spill.cap;
     }
-    if (call_recorded && last_stream != st &&
-        (rc = check(hipStreamWaitEvent(st, call_done, 0), "event wait")))
+    if (c.call_recorded && c.last_stream != st &&
+        (rc = check(hipStreamWaitEvent(st, c.call_done, 0), "event wait")))
         return rc;
 
     BatchDesc hd{};
@@ -279,48 +392,49 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     hd.hap_bytes = v.hap_bytes;
     hd.num_alignments = nA;
     {
-        const int k = desc_next;
-        desc_next = (desc_next + 1) % kDescRing;
-        if (desc_used[k] && (rc = check(hipEventSynchronize(desc_ev[k]), "descriptor slot")))
+        const int k = c.desc_next;
+        c.desc_next = (c.desc_next + 1) % kDescRing;
+        if (c.desc_used[k] && (rc = check(hipEventSynchronize(c.desc_ev[k]), "descriptor slot")))
             return rc;
-        h_desc[k] = hd;
-        if ((rc = check(hipMemcpyAsync(desc.p, &h_desc[k], sizeof(BatchDesc),
+        c.h_desc[k] = hd;
+        if ((rc = check(hipMemcpyAsync(c.desc.p, &c.h_desc[k], sizeof(BatchDesc),
                                        hipMemcpyHostToDevice, st),
                         "descriptor copy")) ||
-            (rc = check(hipEventRecord(desc_ev[k], st), "event")))
+            (rc = check(hipEventRecord(c.desc_ev[k], st), "event")))
             return rc;
-        desc_used[k] = true;
+        c.desc_used[k] = true;
     }
 
     // counters: [0] max hap length, [1] final-list count, [2] error flag,
     // [4 .. 4+NB) per-bin rescue counts, [128 .. 128+kTaskCounters) task
     // counters of the set launches, [256 .. 256+kHmaxParts) partial maxima
     static_assert(4 + kMaxBins <= 128 && 128 + kTaskCounters <= 256, "counter layout");
-    int32_t* sm = static_cast<int32_t*>(smallints.p);
+    int32_t* sm = static_cast<int32_t*>(c.smallints.p);
     BinParams bp{};
-    bp.bd = static_cast<const BatchDesc*>(desc.p);
+    bp.bd = static_cast<const BatchDesc*>(c.desc.p);
     bp.NB = NB;
     bp.NW = NW;
-    bp.rows = static_cast<uint32_t*>(rows.p);
-    bp.hapc = static_cast<uint8_t*>(hapc.p);
-    bp.meta = static_cast<ReadMeta*>(meta.p);
-    bp.gb_count = static_cast<int32_t*>(gb_count.p);
-    bp.gb_start = static_cast<int32_t*>(gb_start.p);
-    bp.read_perm = static_cast<int32_t*>(read_perm.p);
-    bp.align_off = static_cast<int64_t*>(align_off.p);
-    bp.key_count = static_cast<int32_t*>(key_count.p);
-    bp.key_cursor = static_cast<int32_t*>(key_cursor.p);
-    bp.bin_range = static_cast<int32_t*>(bin_range.p);
+    bp.rows = static_cast<uint32_t*>(sp[kRows]);
+    bp.hapc = static_cast<uint8_t*>(sp[kHapc]);
+    bp.meta = static_cast<ReadMeta*>(sp[kMeta]);
+    bp.gb_count = static_cast<int32_t*>(sp[kGbCount]);
+    bp.gb_start = static_cast<int32_t*>(sp[kGbStart]);
+    bp.read_perm = static_cast<int32_t*>(sp[kReadPerm]);
+    bp.align_off = static_cast<int64_t*>(sp[kAlignOff]);
+    bp.key_count = static_cast<int32_t*>(sp[kKeyCount]);
+    bp.key_cursor = static_cast<int32_t*>(sp[kKeyCursor]);
+    bp.bin_range = static_cast<int32_t*>(sp[kBinRange]);
     bp.hmax = sm + 0;
     bp.hmax_part = sm + 256;
     bp.final_count = sm + 1;
     bp.error_flag = sm + 2;
     bp.rescue_counts = sm + 4;
     bp.task_ctr = sm + 128;
-    bp.hapflag = static_cast<uint8_t*>(hapflag.p);
-    bp.items = static_cast<WorkItem*>(items.p);
-    bp.st = static_cast<SuperTask*>(supertasks.p);
-    bp.align_range = static_cast<int32_t*>(align_range.p);
+    bp.hapflag = static_cast<uint8_t*>(sp[kHapFlag]);
+    bp.rescue_hist = static_cast<int32_t*>(sp[kRescueKeys]);
+    bp.items = static_cast<WorkItem*>(sp[kItems]);
+    bp.st = static_cast<SuperTask*>(sp[kSuperTasks]);
+    bp.align_range = static_cast<int32_t*>(sp[kAlignRange]);
     bp.align_part = bp.align_range + 2 * NB;
     bp.ph2pr = d_ph2pr;
     bp.ntiles = ntiles;
@@ -339,15 +453,15 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     fp.read_perm = bp.read_perm;
     fp.hapflag = bp.hapflag;
     fp.ph2pr = d_ph2pr;
-    fp.rescue_items = static_cast<WorkItem*>(rescue_items.p);
+    fp.rescue_items = static_cast<WorkItem*>(sp[kRescueItems]);
     fp.rescue_counts = bp.rescue_counts;
-    fp.final_items = static_cast<WorkItem*>(final_items.p);
+    fp.final_items = static_cast<WorkItem*>(sp[kFinalItems]);
     fp.final_count = bp.final_count;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;
 
-    const bool timed = opt.time_kernels != 0;
+    const bool timed = opt.time_kernels != 0 && !arena;
     std::pair<hipEvent_t, hipEvent_t> evp{nullptr, nullptr};
     if (timed) {
         if (ev_free.empty()) {
@@ -360,17 +474,17 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         }
     }
     const bool plain = opt.mode == PHMM_MODE_PLAIN;
-    if (plain || !opt.graphs) {
+    if (plain || !opt.graphs || arena) {
         if ((rc = enqueue_prep(bp, hd, false, st)))
             return rc;
         if (plain) {
             // one host round trip per batch for the per-bin sizes (the
             // reference syncs per chunk for its per-partition counts,
             // gpuworker.cuh:852-854)
-            if ((rc = check(hipMemcpyAsync(h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NW,
+            if ((rc = check(hipMemcpyAsync(c.h_bin_range, bp.bin_range, sizeof(int32_t) * 2 * NW,
                                            hipMemcpyDeviceToHost, st),
                             "bin range readback")) ||
-                (rc = check(hipMemcpyAsync(h_bin_range + 2 * kMaxWorkBins, bp.align_range,
+                (rc = check(hipMemcpyAsync(c.h_bin_range + 2 * kMaxWorkBins, bp.align_range,
                                            sizeof(int32_t) * 2 * NB, hipMemcpyDeviceToHost, st),
                             "bin range readback")) ||
                 (rc = check(hipStreamSynchronize(st), "bin range sync")))
@@ -387,10 +501,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     } else {
         // batched: replay captured graphs -- one for the whole call, or one
         // per stage when the forward kernels are timed
-        Graphs& graphs = this->graphs[act == &tiles_small ? 1 : 0];
-        if (graphs.gen != generation) {
-            drop_graphs();
-            this->graphs[0].gen = this->graphs[1].gen = generation;
+        Graphs& graphs = c.graphs[act == &tiles_small ? 1 : 0];
+        if (graphs.gen != c.generation) {
+            drop_graphs(c);
+            c.graphs[0].gen = c.graphs[1].gen = c.generation;
         }
         auto launch = [&](int stages, hipGraphExec_t* ex) -> int {
             int r2;
@@ -415,10 +529,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     }
     if (timed)
         ev_pending.push_back(evp);
-    if ((rc = check(hipEventRecord(call_done, st), "event")))
+    if ((rc = check(hipEventRecord(c.call_done, st), "event")))
         return rc;
-    last_stream = st;
-    call_recorded = true;
+    c.last_stream = st;
+    c.call_recorded = true;
     stats.alignments += nA;
     return PHMM_OK;
 }
@@ -440,8 +554,9 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
     const int ntiles = (int)tiles.size();
     const int mb = 2 * ntiles;  // multi-tile bin
     const int eb = mb + 1;      // exact bin
-    float* spill32 = static_cast<float*>(spill.p);
-    const int64_t n32 = (int64_t)(spill.cap / sizeof(float));
+    float* spill32 = static_cast<float*>(cur->call_spill);
+    const int64_t n32 = (int64_t)(cur->call_spill_bytes / sizeof(float));
+    const int32_t* h_bin_range = cur->h_bin_range;
     const int32_t* h_align = h_bin_range + 2 * kMaxWorkBins;
     const int NB = bp.NB;  // item bin of tile bin b: NB + b
     int rc;
@@ -473,9 +588,9 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             if (ctr >= kTaskCounters - kF64Sets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
-            const int grid =
-                std::min(tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false));
-            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fp, bs, grid, st),
+            const int grid = std::min(
+                tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false, roweps()));
+            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fp, bs, grid, st, roweps()),
                             "forward kernel")))
                 return rc;
         }
@@ -503,7 +618,8 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             if (ctr >= kTaskCounters - kF64Sets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
-            if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st), "forward kernel")))
+            if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st, roweps()),
+                            "forward kernel")))
                 return rc;
         }
     }
@@ -537,9 +653,9 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
 {
     const std::vector<phmm_tile_config>& tiles = *act;
     const int ntiles = (int)tiles.size();
-    double* spill64 = static_cast<double*>(spill.p);
-    const int64_t n64 = (int64_t)(spill.cap / sizeof(double));
-    const int32_t* h_align = h_bin_range + 2 * kMaxWorkBins;
+    double* spill64 = static_cast<double*>(cur->call_spill);
+    const int64_t n64 = (int64_t)(cur->call_spill_bytes / sizeof(double));
+    const int32_t* h_align = cur->h_bin_range + 2 * kMaxWorkBins;
     int rc;
     // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
     // the bin's form: FAST-eligible reads recompute in the FAST form).  Each
@@ -551,17 +667,34 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     // the widest tile of each form for every bin: a small batch's FP64 time
     // is the tail of each set launch, so one set per form beats a set per
     // cost choice.
+    // each bin's rescue list in descending hap-length order, into the FP32
+    // pass's item list (no longer read): FP64 tasks of equal-length sweeps
+    {
+        RescueSort rs{};
+        rs.bd = bp.bd;
+        rs.src = fp.rescue_items;
+        rs.dst = bp.items;
+        rs.counts = fp.rescue_counts;
+        rs.range = bp.align_range;
+        rs.hist = bp.rescue_hist;
+        rs.cursor = bp.rescue_hist + kMaxBins * kRescueBuckets;
+        rs.nbins = 2 * ntiles;
+        if ((rc = check(run_rescue_sort(rs, !plain, act == &tiles_small, st), "rescue sort")))
+            return rc;
+    }
     static constexpr int kNr64[] = {4, 6, 8, 10};
     const bool one_set = act == &tiles_small;
+    // per-position gcp: the 8-row FP64 tiles in both forms
+    const int one_set_fast = roweps() ? kRowEpsRows64 : 10;
     BinSet set[4][2], wave;  // [rows index][SAFE, FAST]
     for (auto& row : set)
         for (BinSet& x : row) {
-            x.items = fp.rescue_items;
+            x.items = bp.items;
             x.range = bp.align_range;
             x.counts = fp.rescue_counts;
             x.count_stride = 1;
         }
-    wave.items = fp.rescue_items;
+    wave.items = bp.items;
     wave.range = bp.align_range;
     wave.counts = fp.rescue_counts;
     wave.count_stride = 1;
@@ -583,7 +716,7 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
             const int G = (tile + nr - 1) / nr;
             if (G > 64 || (!fast && nr > 8))
                 continue;
-            if (one_set && nr != (fast ? 10 : 8))
+            if (one_set && nr != (fast ? one_set_fast : 8))
                 continue;
             const double cost = (64.0 / (64 / G)) * ((fast ? 5 : 6) * nr + 12);
             if (best < 0 || cost < best_cost) {
@@ -609,7 +742,7 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
             if (!x.n)
                 continue;
             x.next_task = bp.task_ctr + (kTaskCounters - kF64Sets + nset++);
-            if ((rc = check(launch_fwd_tile_set_f64(kNr64[i], f == 1, fp, x, 0, st),
+            if ((rc = check(launch_fwd_tile_set_f64(kNr64[i], f == 1, fp, x, 0, st, roweps()),
                             "FP64 tile kernel")))
                 return rc;
         }
@@ -682,8 +815,9 @@ void Engine::derive_small_tiles()
     }
 }
 
-void Engine::drop_graphs()
+void Engine::drop_graphs(CallCtx& c)
 {
+    Graphs* graphs = c.graphs;
     hipGraphExec_t* all[] = {&graphs[0].all, &graphs[0].prep, &graphs[0].fwd, &graphs[0].rescue,
                              &graphs[1].all, &graphs[1].prep, &graphs[1].fwd, &graphs[1].rescue};
     bool any = false;
@@ -783,8 +917,8 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
         L.total = o;
     }
     int rc;
-    if ((rc = ensure_pin(s.pin, L.total, "staging")) || (rc = ensure(s.dev, L.total, "chunk")) ||
-        (rc = ensure(s.scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1), "scores")) ||
+    if ((rc = ensure_pin(s.pin, L.total, "staging")) || (rc = grow(s.dev, L.total, "chunk")) ||
+        (rc = grow(s.scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1), "scores")) ||
         (rc = ensure_pin(s.pin_scores, sizeof(float) * (size_t)std::max<int64_t>(nA, 1),
                          "score staging")))
         return rc;
@@ -814,21 +948,29 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
     rp[0] = 0;
     hp[0] = 0;
     // the bulk bytes as slices of at most 2 MiB, spread over host threads
-    // for large chunks; the per-sequence bookkeeping per part
+    // for large chunks; the per-sequence bookkeeping per part.  A caller's
+    // array in pinned memory (the reference's batches are, common.cuh:
+    // 115-116) is not staged: it is copied to the device directly.
     struct Copy {
         char* dst;
         const void* src;
         size_t n;
     };
     std::vector<Copy> copies;
+    struct H2D {
+        size_t off;       // in the chunk layout (device and staging alike)
+        const void* src;  // a pinned caller array, or null: the staging copy
+        size_t n;
+    };
+    std::vector<H2D> h2d;
     for (int k = 0; k < nparts; ++k) {
         const phmm_batch_view& v = *parts[k].v;
         const Span& x = sp[(size_t)k];
         const Base& bs = base[(size_t)k];
         const size_t rb = (size_t)(x.rb1 - x.rb0), hb = (size_t)(x.hb1 - x.hb0);
-        // gap-continuation qualities are read only under CPU semantics (the
-        // GPU kernel's eps is fixed): the other semantics skip their copy
-        const size_t gb = opt.semantics == PHMM_SEM_CPU ? rb : 0;
+        // gap-continuation qualities are read only under the CPU and
+        // per-position gcp semantics (the GPU kernel's eps is fixed)
+        const size_t gb = opt.semantics != PHMM_SEM_GPU ? rb : 0;
         const Copy whole[6] = {{P + L.read_data + bs.rb, v.read_data + x.rb0, rb},
                                {P + L.bq + bs.rb, v.base_quals + x.rb0, rb},
                                {P + L.iq + bs.rb, v.ins_quals + x.rb0, rb},
@@ -836,11 +978,21 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
                                {P + L.gq + bs.rb, v.gcp_quals + x.rb0, gb},
                                {P + L.hap_data + bs.hb, v.hap_data + x.hb0, hb}};
         constexpr size_t kSlice = (size_t)2 << 20;
-        for (const Copy& c : whole)
+        for (const Copy& c : whole) {
+            if (!c.n)
+                continue;
+            const size_t off = (size_t)(c.dst - P);
+            if (host_pinned(c.src)) {
+                h2d.push_back({off, c.src, c.n});
+                continue;
+            }
+            h2d.push_back({off, nullptr, c.n});
             for (size_t o = 0; o < c.n; o += kSlice)
                 copies.push_back({c.dst + o, static_cast<const char*>(c.src) + o,
                                   std::min(kSlice, c.n - o)});
+        }
     }
+    h2d.push_back({(size_t)L.rl, nullptr, (size_t)(L.total - L.rl)});  // lengths, offsets, groups
     auto bookkeeping = [&](int k) {
         const phmm_batch_view& v = *parts[k].v;
         const int g0 = parts[k].g0, g1 = parts[k].g1;
@@ -890,20 +1042,28 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
             (rc = check(hipEventCreateWithFlags(&s.compute_done, hipEventDisableTiming), "event")))
             return rc;
     }
-    // one copy of the staged chunk; without CPU semantics the gcp region
-    // (between the deletion qualities and the haps) stays behind
-    if (opt.semantics == PHMM_SEM_CPU) {
-        rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.total, hipMemcpyHostToDevice, copy_stream),
-                   "H2D copy");
-    } else {
-        rc = check(hipMemcpyAsync(s.dev.p, s.pin.p, L.gq, hipMemcpyHostToDevice, copy_stream),
-                   "H2D copy");
-        if (!rc)
-            rc = check(hipMemcpyAsync(static_cast<char*>(s.dev.p) + L.hap_data,
-                                      static_cast<char*>(s.pin.p) + L.hap_data,
-                                      L.total - L.hap_data, hipMemcpyHostToDevice, copy_stream),
-                       "H2D copy");
+    // the copies in layout order, neighbouring staged ranges as one (the
+    // gaps between arrays are alignment padding); under the GPU kernel's
+    // semantics the gcp region was not staged and stays behind
+    std::sort(h2d.begin(), h2d.end(), [](const H2D& a, const H2D& b) { return a.off < b.off; });
+    {
+        std::vector<H2D> merged;
+        for (const H2D& c : h2d) {
+            if (!merged.empty() && !c.src && !merged.back().src &&
+                c.off <= merged.back().off + merged.back().n + 256)
+                merged.back().n = c.off + c.n - merged.back().off;
+            else
+                merged.push_back(c);
+        }
+        h2d.swap(merged);
     }
+    rc = PHMM_OK;
+    for (const H2D& c : h2d)
+        if ((rc = check(hipMemcpyAsync(static_cast<char*>(s.dev.p) + c.off,
+                                       c.src ? c.src : static_cast<const void*>(P + c.off), c.n,
+                                       hipMemcpyHostToDevice, copy_stream),
+                        "H2D copy")))
+            break;
     if (rc ||
         (rc = check(hipEventRecord(s.h2d_done, copy_stream), "event")) ||
         (rc = check(hipStreamWaitEvent(stream, s.h2d_done, 0), "event wait")))
@@ -972,6 +1132,32 @@ int Engine::score_batch(const phmm_batch_view& v, float* out)
         }
         if (g0 < v.num_groups)
             chunks.push_back({g0, v.num_groups});
+    }
+    // Pieces of a transfer chunk: a large chunk is staged, copied and scored
+    // in up to four record-aligned pieces, so the copies of piece k+1 overlap
+    // the kernels of piece k (scores do not depend on the cut).
+    {
+        std::vector<std::pair<int, int>> pieces;
+        for (const auto& ch : chunks) {
+            int64_t na = 0;
+            for (int g = ch.first; g < ch.second; ++g)
+                na += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+            const int n = na >= ((int64_t)1 << 19) ? 4 : na >= ((int64_t)1 << 17) ? 2 : 1;
+            int g0 = ch.first;
+            int64_t acc = 0;
+            int k = 1;
+            for (int g = ch.first; g < ch.second && k < n; ++g) {
+                acc += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+                if (acc * n >= na * k) {
+                    pieces.push_back({g0, g + 1});
+                    g0 = g + 1;
+                    ++k;
+                }
+            }
+            if (g0 < ch.second)
+                pieces.push_back({g0, ch.second});
+        }
+        chunks.swap(pieces);
     }
     int rc;
     if ((rc = take_error_flag()))
@@ -1069,7 +1255,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         phmm_options_default(&o);
     if (o.mode != PHMM_MODE_PLAIN && o.mode != PHMM_MODE_BATCHED)
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown mode");
-    if (o.semantics != PHMM_SEM_GPU && o.semantics != PHMM_SEM_CPU)
+    if (o.semantics != PHMM_SEM_GPU && o.semantics != PHMM_SEM_CPU && o.semantics != PHMM_SEM_GCP)
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown semantics");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -1104,11 +1290,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
         (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipHostMalloc(&e.h_bin_range,
-                                    2 * sizeof(int32_t) * (phmm::kMaxWorkBins + phmm::kMaxBins),
-                                    hipHostMallocDefault),
-                      "pinned")) ||
-        (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")))
+        (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")) ||
+        (rc = e.init_ctx(0)))
         return bail(rc);
     // ph2pr[q] = (float)10^(-q/10) (gpuworker.cuh:473-486)
     float table[128];
@@ -1184,7 +1367,8 @@ extern "C" int phmm_engine_set_tiles(phmm_engine* h, const phmm_tile_config* t, 
     std::lock_guard<std::mutex> lk(h->mu);
     h->e.tiles.assign(t, t + n);
     h->e.derive_small_tiles();
-    ++h->e.generation;  // captured graphs hold the old table
+    for (auto& c : h->e.ctxs)
+        ++c.generation;  // captured graphs hold the old table
     return PHMM_OK;
 }
 
@@ -1337,6 +1521,28 @@ extern "C" int phmm_device_free(phmm_engine* h, void* p)
         hipDeviceSynchronize();
         hipFree(p);
     }
+    return PHMM_OK;
+}
+
+extern "C" int phmm_host_alloc_pinned(phmm_engine* h, size_t bytes, void** out)
+{
+    if (!h || !out)
+        return phmm::fail(PHMM_EINVAL, "phmm_host_alloc_pinned: null argument");
+    hipSetDevice(h->e.device);
+    if (hipHostMalloc(out, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+        hipGetLastError();
+        *out = nullptr;
+        return phmm::fail(PHMM_ENOMEM, "phmm_host_alloc_pinned: pinned allocation failed");
+    }
+    return PHMM_OK;
+}
+
+extern "C" int phmm_host_free_pinned(phmm_engine* h, void* p)
+{
+    if (!h)
+        return phmm::fail(PHMM_EINVAL, "phmm_host_free_pinned: null engine");
+    if (p)
+        hipHostFree(p);
     return PHMM_OK;
 }
 

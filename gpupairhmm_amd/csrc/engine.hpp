@@ -35,45 +35,88 @@ struct Engine {
     // classes of `tiles`, 16 rows per lane each (where 64 lanes hold them).
     std::vector<phmm_tile_config> tiles_small;
     const std::vector<phmm_tile_config>* act = &tiles;  // the current call's table
+    // per-position gcp: the tile kernels with a gap continuation per row
+    bool roweps() const { return opt.semantics == PHMM_SEM_GCP; }
     void derive_small_tiles();
     float* d_ph2pr = nullptr;
     int num_cus = 256;
     std::string err;
     phmm_stats stats{};
 
-    // scratch (grow-only)
-    DevBuf rows, hapc, meta, gb_count, gb_start, read_perm, align_off, key_count, key_cursor,
-        bin_range, align_range, smallints, items, supertasks, rescue_items, final_items, spill,
-        hapflag;
-    int32_t* h_bin_range = nullptr;   // pinned: 2*kMaxWorkBins work ranges, then 2*kMaxBins
-                                      // alignment ranges
+    // Batch-sized scratch of a call, in this order (grow-only buffers, or
+    // carved from a caller's arena by the allocation-free entry point).
+    enum ScratchBuf {
+        kRows, kHapc, kMeta, kGbCount, kGbStart, kReadPerm, kAlignOff, kKeyCount, kKeyCursor,
+        kBinRange, kItems, kAlignRange, kSuperTasks, kRescueItems, kFinalItems, kHapFlag,
+        kRescueKeys, kNumScratch
+    };
+    // An arena must leave at least this much for the spill arena (one block
+    // of the FP64 wave kernel holds 32 bytes per column of the longest hap).
+    static constexpr size_t kMinArenaSpill = (size_t)8 << 20;
+    static constexpr int kDescRing = 16;
+    struct Graphs {
+        uint64_t gen = ~0ull;
+        hipGraphExec_t all = nullptr, prep = nullptr, fwd = nullptr, rescue = nullptr;
+    };
+    // The device state of a call: batch-sized scratch, counters, the batch
+    // descriptor and the captured graphs that hold their addresses.  Calls of
+    // one context share it and run one after another (a call on another
+    // stream than the previous one waits for it); calls of different
+    // contexts may overlap on the device.  Context 0 serves the C ABI's
+    // score calls; the serving loop drives further ones on its own streams.
+    struct CallCtx {
+        DevBuf sbuf[kNumScratch];
+        DevBuf spill;                 // border rows of the one-wave kernels
+        size_t spill_bytes = (size_t)256 << 20;
+        // fixed-size state, allocated by init_ctx: counters (zeroed) and the
+        // device copy of the batch descriptor, staged through a ring of
+        // pinned slots (a slot is rewritten only after its copy ran)
+        DevBuf smallints;
+        DevBuf desc;
+        BatchDesc* h_desc = nullptr;
+        hipEvent_t desc_ev[kDescRing] = {};
+        bool desc_used[kDescRing] = {};
+        int desc_next = 0;
+        int32_t* h_bin_range = nullptr;  // pinned: 2*kMaxWorkBins work ranges, then 2*kMaxBins
+                                         // alignment ranges (plain mode)
+        hipEvent_t call_done = nullptr;
+        hipStream_t last_stream = nullptr;
+        bool call_recorded = false;
+        bool ready = false;
+        // Kernel arguments hold scratch pointers and the tile table, so every
+        // reallocation or table change starts a new generation and the graphs
+        // are captured again.
+        uint64_t generation = 0;
+        Graphs graphs[2];  // [0]: tuned table, [1]: small-batch table
+        // the current call's spill arena (spill, or the rest of an arena)
+        void* call_spill = nullptr;
+        size_t call_spill_bytes = 0;
+    };
+    static constexpr int kMaxCtx = 4;
+    CallCtx ctxs[kMaxCtx];
+    bool claimed[kMaxCtx] = {true};  // context 0: the C ABI's score calls
+    // A free context (>= 1) for a serving loop, or -1; release returns it.
+    int claim_ctx()
+    {
+        for (int k = 1; k < kMaxCtx; ++k)
+            if (!claimed[k]) {
+                claimed[k] = true;
+                return k;
+            }
+        return -1;
+    }
+    void release_ctx(int k)
+    {
+        if (k > 0 && k < kMaxCtx)
+            claimed[k] = false;
+    }
+    CallCtx* cur = &ctxs[0];  // the context of the call being enqueued
     int32_t* h_small = nullptr;       // pinned, small readbacks
     // event pairs around the forward kernels of timed calls
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_pending;
 
     int64_t* d_rescued_total = nullptr;  // cumulative FP64 rescues
 
-    // The per-call batch descriptor: one device copy, staged through a ring
-    // of pinned slots (a slot is rewritten only after its copy ran).
-    static constexpr int kDescRing = 16;
-    DevBuf desc;
-    BatchDesc* h_desc = nullptr;
-    hipEvent_t desc_ev[kDescRing] = {};
-    bool desc_used[kDescRing] = {};
-    int desc_next = 0;
-    // Calls share the scratch buffers: a call on another stream than the
-    // previous one waits for it.
-    hipEvent_t call_done = nullptr;
-    hipStream_t last_stream = nullptr;
-    bool call_recorded = false;
-    // Captured graphs of the batched kernel sequence.  Kernel arguments hold
-    // scratch pointers and the tile table, so every reallocation or table
-    // change starts a new generation and the graphs are captured again.
-    uint64_t generation = 0;
-    struct Graphs {
-        uint64_t gen = ~0ull;
-        hipGraphExec_t all = nullptr, prep = nullptr, fwd = nullptr, rescue = nullptr;
-    } graphs[2];  // [0]: tuned table, [1]: small-batch table
     hipStream_t cap_stream = nullptr;
     int64_t graph_captures = 0, graph_launches = 0;
 
@@ -102,10 +145,28 @@ struct Engine {
         return set_error(e == hipErrorOutOfMemory ? PHMM_ENOMEM : PHMM_EDEVICE,
                          std::string(what) + ": " + hipGetErrorString(e));
     }
-    int ensure(DevBuf& b, size_t bytes, const char* what);
+    // Grows b to at least `bytes`; a reallocation starts a new generation
+    // of context c (its captured graphs hold the old address).
+    int ensure(CallCtx& c, DevBuf& b, size_t bytes, const char* what);
+    // Grows b to at least `bytes` (geometrically; contents not kept).
+    int grow(DevBuf& b, size_t bytes, const char* what);
     int ensure_pin(PinBuf& b, size_t bytes, const char* what);
-    int ensure_zeroed(DevBuf& b, size_t bytes, const char* what);
-    int score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st);
+    // Allocates context k's fixed-size state (counters, descriptor slot and
+    // ring, events) once; context 0 at engine creation.
+    int init_ctx(int k);
+    // Bytes of each batch-sized scratch buffer for view v with NB read bins
+    // and NW work bins, and the (NB, NW) of the table a call of v uses.
+    static void scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[kNumScratch]);
+    void bins_for(const phmm_batch_view& v, int* NB, int* NW) const;
+    // Device bytes context k holds (scratch, spill, counters).
+    size_t ctx_bytes(int k) const;
+    // Grows context k's scratch to hold a call of v (spill included).
+    int reserve_ctx(int k, const phmm_batch_view& v);
+    // arena != null: every batch-sized buffer is carved from the arena_bytes
+    // at arena (no allocation; the kernels launch one by one, untimed).
+    // ctx: the call context (0 for the C ABI's score calls).
+    int score_device(const phmm_batch_view& v, float* d_scores, hipStream_t st,
+                     char* arena = nullptr, size_t arena_bytes = 0, int ctx = 0);
     // The three stages of a call: binning, FP32 forward kernels, FP64
     // rescue.  `plain`: sizes read back to the host and exact launches;
     // else persistent grids that read their ranges from device memory.
@@ -115,7 +176,8 @@ struct Engine {
     enum { kStagePrep = 1, kStageFwd = 2, kStageRescue = 4 };
     int capture(int stages, const BinParams& bp, const FwdParams& fp, const BatchDesc& hd,
                 hipGraphExec_t* out);
-    void drop_graphs();
+    void drop_graphs(CallCtx& c);
+    void free_ctx(CallCtx& c);
     int score_batch(const phmm_batch_view& v, float* out);
     int finish_timing();
     int take_error_flag();

@@ -134,25 +134,61 @@ struct TileTraits<double> {
 
 // Per-lane state of a tile task that depends only on the read: its rows'
 // coefficients (the emission values of the rows live in the LDS table).
-template <typename T, int NR>
+// ROWEPS: every row has its own gap continuation eps_r (per-position gcp
+// semantics), so the D chain takes eps_r from a per-row register (FAST: c3;
+// SAFE: c4) instead of the lane's eps.
+template <typename T, int NR, bool ROWEPS>
 struct TileRead {
-    T c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e -; SAFE: alpha w delta' e
+    T c0[NR], c1[NR], c2[NR], c3[NR];  // FAST: b d e eps_r; SAFE: alpha w delta' e
+    T c4[ROWEPS ? NR : 1];             // SAFE with ROWEPS: eps_r
     T eps, beta;
     int L;
     float nl;
 };
 
+// Gap continuation of the D chain of row j.
+template <typename T, int NR, bool FAST, bool ROWEPS>
+__device__ __forceinline__ T d_eps(const TileRead<T, NR, ROWEPS>& R, int j)
+{
+    if constexpr (!ROWEPS)
+        return R.eps;
+    else if constexpr (FAST)
+        return R.c3[j];
+    else
+        return R.c4[j];
+}
+
+// beta = 1 - eps, rounded as the reference's host code rounds it in FP32
+// (float beta = 1.0 - ph2pr[q], gpuworker.cuh:1048).
+template <typename T>
+__device__ __forceinline__ T beta_of(T e)
+{
+    if constexpr (sizeof(T) == 4)
+        return (T)(float)(1.0 - (double)e);
+    else
+        return (T)1 - e;
+}
+
 // This lane's rows of `read` (inactive lanes: an empty read): coefficients in
 // R, emission values in the LDS table.
-template <typename T, int NR, bool FAST>
+//
+// ROWEPS (per-position gcp): row r (1-based) takes eps_r = ph2pr[gcp[r-1]]
+// and beta_r = 1 - eps_r where the reference takes one eps and beta.  FAST:
+// b_r = beta_r delta_{r-1}/(alpha_{r-1} alpha_r), d_r = beta_r sigma_{r-1} /
+// alpha_r, e_r = eps_r delta_{r-1}/(delta_r alpha_{r-1}), and the D chain
+// Dh = Mh< + eps_r Dh<.  SAFE carries Ib_r = beta_{r+1} I_r (the next row's
+// beta, so row r+1's M adds it with weight 1): c2 = beta_{r+1} delta_r,
+// c3 = eps_r beta_{r+1} / beta_r, and D chain Dh = M< + eps_r Dh<; the last
+// row's beta_{L+1} is taken as beta_L, which the epilogue divides out.
+template <typename T, int NR, bool FAST, bool ROWEPS>
 __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTraits<T>::V* tab,
-                                           bool active, int read, int t, TileRead<T, NR>& R)
+                                           bool active, int read, int t,
+                                           TileRead<T, NR, ROWEPS>& R)
 {
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
     constexpr int K = TT::K;
     constexpr int NQ = (NR + K - 1) / K;  // the last vector may hold fewer than K rows
-    constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     int L = 0, roff = 0;
     float nl = 1.0f;
@@ -163,15 +199,19 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
             nl = p.meta[read].nl;
     }
     T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
-    if (p.semantics == 1 && active && L > 0) {
-        const float e = ph2pr_of(p.ph2pr, bd.gcp[roff]);
+    if (p.semantics != 0 && active && L > 0) {
+        // reference CPU semantics: the read's first gcp quality; per-position
+        // gcp: the last row's (the SAFE epilogue divides by beta_{L+1} = beta_L)
+        const float e = ph2pr_of(p.ph2pr, bd.gcp[roff + (ROWEPS ? L - 1 : 0)]);
         eps = (T)e;
-        beta = F32 ? (T)(float)(1.0 - (double)e) : (T)1 - (T)e;
+        beta = beta_of((T)e);
     }
     R.eps = eps;
     R.beta = beta;
     R.L = L;
     R.nl = nl;
+    // per-position gcp: eps and beta of row index ri (0-based, < L)
+    auto row_eps = [&](int ri) -> T { return (T)ph2pr_of(p.ph2pr, bd.gcp[roff + ri]); };
 
     // ---- per-row coefficients and the emission table -------------------
     const int r0 = t * NR;
@@ -194,25 +234,35 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
                 const T delta = (T)ph2pr_of(p.ph2pr, row_iq(w));
                 const T sigma = (T)ph2pr_of(p.ph2pr, row_dq(w));
                 const T alpha = (T)1 - (delta + sigma);
+                // gap continuation of this row, and (SAFE) the next row's beta
+                T er = eps, br = beta, bn = beta;
+                if constexpr (ROWEPS) {
+                    er = row_eps(ri);
+                    br = beta_of(er);
+                    bn = ri + 1 < L ? beta_of(row_eps(ri + 1)) : beta;
+                }
                 code[j] = row_code(w);
                 lm[j] = (T)1 - pb;
                 lx[j] = qdiv(pb, (T)3);
                 if (FAST) {
                     if (ri == 0) {
                         R.c0[j] = 0;
-                        R.c1[j] = qdiv(beta, alpha);
+                        R.c1[j] = qdiv(br, alpha);
                         R.c2[j] = 0;
                     } else {
-                        R.c0[j] = qdiv(beta * pdelta, palpha * alpha);
-                        R.c1[j] = qdiv(beta * psigma, alpha);
-                        R.c2[j] = qdiv(eps * pdelta, delta * palpha);
+                        R.c0[j] = qdiv(br * pdelta, palpha * alpha);
+                        R.c1[j] = qdiv(br * psigma, alpha);
+                        R.c2[j] = qdiv(er * pdelta, delta * palpha);
                     }
-                    R.c3[j] = 0;
+                    R.c3[j] = ROWEPS ? er : (T)0;
                 } else {
                     R.c0[j] = alpha;
-                    R.c1[j] = ri == 0 ? beta : beta * psigma;
-                    R.c2[j] = beta * delta;
-                    R.c3[j] = eps;
+                    R.c1[j] = ri == 0 ? br : br * psigma;
+                    R.c2[j] = bn * delta;
+                    // equal betas keep eps exactly (no rounding of bn / br)
+                    R.c3[j] = bn == br ? er : qdiv(er * bn, br);
+                    if constexpr (ROWEPS)
+                        R.c4[j] = er;
                 }
                 pdelta = delta;
                 psigma = sigma;
@@ -232,6 +282,8 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
                     R.c1[j] = 0;
                     R.c2[j] = firstoob ? beta : (T)0;
                     R.c3[j] = 1;
+                    if constexpr (ROWEPS)
+                        R.c4[j] = 0;
                 }
             }
         }
@@ -267,11 +319,11 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
 // + n], n counted in rescue_counts[bin]).  T = double is the FP64 recompute
 // of those lists (start constant 2^1000/H); a result whose sum is not far
 // above double's floor goes to the final list.
-template <typename T, int NR, bool FAST>
+template <typename T, int NR, bool FAST, bool ROWEPS>
 __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
-                                           const TileRead<T, NR>& R, bool active, int G, int t,
-                                           int HL, int hoff, const WorkItem& it, int rescue_base,
-                                           int bin)
+                                           const TileRead<T, NR, ROWEPS>& R, bool active, int G,
+                                           int t, int HL, int hoff, const WorkItem& it,
+                                           int rescue_base, int bin)
 {
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
@@ -281,7 +333,7 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     const bool first = (t == 0);
-    const T eps = R.eps, beta = R.beta;
+    const T beta = R.beta;
     const int L = R.L;
     const float nl = R.nl;
 
@@ -382,7 +434,7 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
                         chain = fma(R.c2[j], mp, R.c3[j] * chain);
                     }
                     M[j] = lv[k] * t1;
-                    D[j] = fma(eps, dov, mo);
+                    D[j] = fma((d_eps<T, NR, FAST, ROWEPS>(R, j)), dov, mo);
                     mp = mo;
                     dp = dov;
                 }
@@ -440,7 +492,7 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
 
 // A task of 64/G consecutive alignments of an item list (the FP64 recompute
 // of rescue lists): every group sets up its own read and sweeps its own hap.
-template <typename T, int NR, bool FAST>
+template <typename T, int NR, bool FAST, bool ROWEPS>
 __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const WorkItem* items, int begin, int count, int G,
                                                 int task, int rescue_base, int bin)
@@ -459,18 +511,18 @@ __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename Til
     it.pad = 0;
     if (active)
         it = items[begin + local];
-    TileRead<T, NR> R;
-    tile_setup<T, NR, FAST>(p, tab, active, it.read, t, R);
+    TileRead<T, NR, ROWEPS> R;
+    tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, it.read, t, R);
     const int HL = active ? bd.hap_len[it.hap] : 0;
     const int hoff = active ? bd.hap_off[it.hap] : 0;
-    tile_sweep<T, NR, FAST>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
+    tile_sweep<T, NR, FAST, ROWEPS>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
 }
 
 // A super-task: the groups set up their reads once and sweep the segment's
 // haps in turn (all groups share each hap, so every sweep is uniform across
 // the wave).  Empty haps were scored -inf by the scatter; haps holding a
 // byte other than A/C/G/T belong to the exact bin.
-template <typename T, int NR, bool FAST>
+template <typename T, int NR, bool FAST, bool ROWEPS>
 __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const SuperTask& st, int G, int rescue_base, int bin)
 {
@@ -481,8 +533,8 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
     const int t = lane - grp * G;
     const bool active = grp < W && grp < st.nc;
     const int rd = active ? p.read_perm[st.perm + grp] : 0;
-    TileRead<T, NR> R;
-    tile_setup<T, NR, FAST>(p, tab, active, rd, t, R);
+    TileRead<T, NR, ROWEPS> R;
+    tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, rd, t, R);
     for (int h = st.hs; h < st.he; ++h) {
         const int hap = st.h0 + h;
         const int HL = bd.hap_len[hap];
@@ -493,7 +545,8 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
         it.hap = hap;
         it.out = st.aoff + (rd - st.r0) * st.nh + h;
         it.pad = 0;
-        tile_sweep<T, NR, FAST>(p, tab, R, active, G, t, HL, bd.hap_off[hap], it, rescue_base, bin);
+        tile_sweep<T, NR, FAST, ROWEPS>(p, tab, R, active, G, t, HL, bd.hap_off[hap], it,
+                                        rescue_base, bin);
     }
 }
 
@@ -553,7 +606,7 @@ __device__ __forceinline__ int next_task(int32_t* ctr)
 // Bins of one template in one persistent launch.  Waves take tasks in order
 // from a counter: consecutive tasks of a length-sorted bin have similar
 // lengths, and a wave that finishes early takes the next one.
-template <typename T, int NR, bool FAST>
+template <typename T, int NR, bool FAST, bool ROWEPS = false>
 __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
 {
     static_assert(NR >= 4 && NR <= 32, "rows per lane");
@@ -574,10 +627,11 @@ __global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
         const int rbin = bin >= bs.item_bin0 ? bin - bs.item_bin0 : bin;
         const int rescue_base = bs.seg ? bs.seg[2 * rbin] : 0;
         if (bs.st && bin < bs.item_bin0)
-            tile_super_task<T, NR, FAST>(p, tab, bs.st[begin + (task - t0)], G, rescue_base, rbin);
+            tile_super_task<T, NR, FAST, ROWEPS>(p, tab, bs.st[begin + (task - t0)], G,
+                                                 rescue_base, rbin);
         else
-            tile_items_task<T, NR, FAST>(p, tab, bs.items, begin, count, G, task - t0, rescue_base,
-                                         rbin);
+            tile_items_task<T, NR, FAST, ROWEPS>(p, tab, bs.items, begin, count, G, task - t0,
+                                                 rescue_base, rbin);
     }
 }
 
@@ -665,24 +719,28 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
             }
             continue;
         }
+        // gap continuation: fixed (reference GPU semantics), the read's first
+        // gcp quality (reference CPU semantics) or every row's own
+        // (per-position gcp, set per row below)
         T eps = (T)kEpsGpu, beta = (T)kBetaGpu;
         if (p.semantics == 1) {
             const float e = ph2pr_of(p.ph2pr, bd.gcp[roff]);
             eps = (T)e;
             beta = (T)1 - (T)e;
         }
+        const bool row_gcp = p.semantics == 2;
         const T C0 = S::ldexp((T)1, 64) / (T)HL;
         const int ntiles = (L + 1 + TR - 1) / TR;
         // Emission match: codes with 4 as the wildcard (reference GPU
         // semantics), or the raw bytes with a literal 'N' as the wildcard
         // (reference CPU semantics, gpuworker.cuh:1063) -- exact for any byte.
-        const bool raw = p.semantics == 1;
+        const bool raw = p.semantics != 0;
         const int wild = raw ? (int)'N' : 4;
         T acc = 0;
         int sc = 0;
         for (int k = 0; k < ntiles; ++k) {
             const int r0 = k * TR + lane * NR;
-            T ca[NR], cdl[NR], cs[NR], ce[NR], lm[NR], lx[NR];
+            T ca[NR], cb[NR], cdl[NR], cs[NR], ce[NR], lm[NR], lx[NR];
             int code[NR];
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
@@ -695,12 +753,20 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
                     ca[j] = (T)1 - (de + si);
                     cdl[j] = de;
                     cs[j] = si;
-                    ce[j] = eps;
+                    if (row_gcp) {
+                        const T e = (T)ph2pr_of(p.ph2pr, bd.gcp[roff + ri]);
+                        ce[j] = e;
+                        cb[j] = (T)1 - e;
+                    } else {
+                        ce[j] = eps;
+                        cb[j] = beta;
+                    }
                     lm[j] = (T)1 - pb;
                     code[j] = raw ? (int)bd.read_data[roff + ri] : row_code(w32);
                     lx[j] = code[j] == wild ? lm[j] : pb / (T)3;
                 } else {
                     ca[j] = 0;
+                    cb[j] = 0;
                     cdl[j] = ri == L ? (T)1 : (T)0;
                     cs[j] = 0;
                     ce[j] = 1;
@@ -780,9 +846,10 @@ __global__ __launch_bounds__(64) void k_fwd_wave(FwdParams p, BinSet bs, const i
                 for (int j = 0; j < NR; ++j) {
                     const T mo = M[j], dov = D[j];
                     const T lam = (letter == code[j] || letter == wild) ? lm[j] : lx[j];
-                    M[j] = lam * (ca[j] * mp + beta * (chain + dp));
+                    M[j] = lam * (ca[j] * mp + cb[j] * (chain + dp));
                     chain = cdl[j] * mp + ce[j] * chain;
-                    D[j] = cs[j] * mo + eps * dov;
+                    // past the read (ce = 1, cs = 0) D stays at its start, 0
+                    D[j] = cs[j] * mo + ce[j] * dov;
                     mp = mo;
                     dp = dov;
                 }
@@ -852,10 +919,10 @@ static int resident_blocks_per_cu(K kernel)
     return n > 0 ? n : 4;
 }
 
-template <typename T, int NR, bool FAST>
+template <typename T, int NR, bool FAST, bool ROWEPS = false>
 static int set_blocks_per_cu()
 {
-    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST>);
+    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST, ROWEPS>);
     return n;
 }
 
@@ -877,7 +944,7 @@ static int grid_of(int grid, int per_cu)
     return cus * per_cu;
 }
 
-template <typename T, int NR>
+template <typename T, int NR, bool ROWEPS = false>
 static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                 hipStream_t st)
 {
@@ -887,20 +954,23 @@ static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs,
         if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
             return hipErrorInvalidValue;
     if (fast) {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true>())), dim3(64), 0, st,
-                           p, bs);
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true, ROWEPS>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true, ROWEPS>())), dim3(64),
+                           0, st, p, bs);
     } else {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false>())), dim3(64), 0, st,
-                           p, bs);
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false, ROWEPS>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false, ROWEPS>())), dim3(64),
+                           0, st, p, bs);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,
-                               hipStream_t st)
+                               hipStream_t st, bool roweps)
 {
+    if (roweps)  // per-position gcp: the 16-row tiles only
+        return NR == kRowEpsRows32 ? launch_set_nr<float, kRowEpsRows32, true>(fast, p, bs, grid, st)
+                                   : hipErrorInvalidValue;
     switch (NR) {
     case 4: return launch_set_nr<float, 4>(fast, p, bs, grid, st);
     case 8: return launch_set_nr<float, 8>(fast, p, bs, grid, st);
@@ -917,8 +987,11 @@ hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinS
 }
 
 hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const BinSet& bs,
-                                   int grid, hipStream_t st)
+                                   int grid, hipStream_t st, bool roweps)
 {
+    if (roweps)  // per-position gcp: the 8-row FP64 tiles only
+        return NR == kRowEpsRows64 ? launch_set_nr<double, kRowEpsRows64, true>(fast, p, bs, grid, st)
+                                   : hipErrorInvalidValue;
     switch (NR) {
     case 4: return launch_set_nr<double, 4>(fast, p, bs, grid, st);
     case 6: return launch_set_nr<double, 6>(fast, p, bs, grid, st);
@@ -928,8 +1001,17 @@ hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const 
     }
 }
 
-int tile_set_blocks_per_cu(int NR, bool fast, bool f64)
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps)
 {
+    if (roweps) {
+        if (!f64 && NR == kRowEpsRows32)
+            return fast ? set_blocks_per_cu<float, kRowEpsRows32, true, true>()
+                        : set_blocks_per_cu<float, kRowEpsRows32, false, true>();
+        if (f64 && NR == kRowEpsRows64)
+            return fast ? set_blocks_per_cu<double, kRowEpsRows64, true, true>()
+                        : set_blocks_per_cu<double, kRowEpsRows64, false, true>();
+        return 0;
+    }
     if (f64) {
         switch (NR) {
         case 4: return fast ? set_blocks_per_cu<double, 4, true>() : set_blocks_per_cu<double, 4, false>();

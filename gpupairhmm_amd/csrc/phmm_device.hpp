@@ -69,6 +69,7 @@ constexpr int kKeyParts = 8;        // counters per (bin, bucket): groups spread
 constexpr int kKeysPerBin = kHapBuckets * kKeyParts;
 constexpr int kAlignParts = 256;    // partial counters per bin (atomic spreading)
 constexpr int kHmaxParts = 512;     // partial maxima of the hap pass (atomic spreading)
+constexpr int kRescueBuckets = 256; // hap-length buckets of the rescue-list sort
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
 __host__ __device__ inline int hap_bucket(int h)
@@ -97,14 +98,15 @@ __host__ __device__ inline int base_code(uint8_t b)
     return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : 4;
 }
 
-// Reference CPU semantics (align_host, gpuworker.cuh:1063): only a literal
-// 'N' is the wildcard; any byte other than A/C/G/T/N gets code 5 and sends
-// its read or hap to the exact bin, whose kernel compares raw bytes.
+// Reference CPU semantics (align_host, gpuworker.cuh:1063), and per-position
+// gcp semantics, which extend them: only a literal 'N' is the wildcard; any
+// byte other than A/C/G/T/N gets code 5 and sends its read or hap to the
+// exact bin, whose kernel compares raw bytes.
 constexpr int kExoticCode = 5;
 __host__ __device__ inline int base_code_sem(uint8_t b, int semantics)
 {
     const int c = base_code(b);
-    return (semantics == 1 && c == 4 && b != 'N') ? kExoticCode : c;
+    return (semantics != 0 && c == 4 && b != 'N') ? kExoticCode : c;
 }
 
 // Per-read metadata from the read pass.
@@ -153,7 +155,7 @@ struct FwdParams {
     int32_t* final_count;
     float rescue_min;            // FP32 sum (reference units) threshold
     int32_t rescue_on;
-    int32_t semantics;           // 0: reference GPU, 1: reference CPU
+    int32_t semantics;           // 0: reference GPU, 1: reference CPU, 2: per-position gcp
     int32_t pad;
 };
 
@@ -183,6 +185,7 @@ struct BinParams {
     int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
     int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin (k_scan_keys sums each bin's into its first)
     uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T
+    int32_t* rescue_hist;      // [2*kMaxBins*kRescueBuckets]: rescue-list bucket counts, then cursors
     WorkItem* items;
     const float* ph2pr;
     // tiles
@@ -200,6 +203,22 @@ struct BinParams {
 hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, bool small,
                        hipStream_t st);
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
+
+// Counting sort of the tile bins' rescue lists (bins 0 .. nbins-1, list of
+// bin b at src + range[2b], counts[b] long) into dst at the same offsets,
+// each list ordered by descending hap-length bucket.  hist (zeroed before)
+// and cursor hold nbins * kRescueBuckets counters.
+struct RescueSort {
+    const BatchDesc* bd;
+    const WorkItem* src;
+    WorkItem* dst;
+    const int32_t* counts;
+    const int32_t* range;
+    int32_t* hist;
+    int32_t* cursor;
+    int32_t nbins;
+};
+hipError_t run_rescue_sort(const RescueSort& rs, bool fixed_grids, bool small, hipStream_t st);
 // Several bins in one persistent launch.  Bin bin[i] holds count_i =
 // counts[count_stride*bin[i]] units starting at begin_i = range ?
 // range[2*bin[i]] : 0, both read on the device.  A unit is a SuperTask
@@ -223,11 +242,16 @@ struct BinSet {
     const int32_t* seg = nullptr;
     int32_t* next_task = nullptr;
 };
-// grid: 0 = as many blocks as fit on the device at once
+// Tile kernels with a gap continuation per row (per-position gcp
+// semantics) exist for these rows per lane only.
+constexpr int kRowEpsRows32 = 16;
+constexpr int kRowEpsRows64 = 8;
+// grid: 0 = as many blocks as fit on the device at once; roweps: the
+// per-position gcp instances (kRowEpsRows32 / kRowEpsRows64 rows only)
 hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,
-                               hipStream_t st);
+                               hipStream_t st, bool roweps = false);
 hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const BinSet& bs,
-                                   int grid, hipStream_t st);
+                                   int grid, hipStream_t st, bool roweps = false);
 hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
                                float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
@@ -235,6 +259,6 @@ hipError_t launch_fwd_wave_f64(const FwdParams& p, const BinSet& bs, const int32
                                double* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
 // Resident blocks of 64 threads per CU for a tile-set instance.
-int tile_set_blocks_per_cu(int NR, bool fast, bool f64);
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps = false);
 
 }  // namespace phmm

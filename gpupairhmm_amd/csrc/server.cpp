@@ -1,18 +1,33 @@
-// Serving loop: continuous batching of active-region requests.
+// Serving loop: continuous batching of active-region requests on one GPU.
 //
-// Requests (each a small host batch, usually one region = one record) are
-// admitted while their estimated device footprint fits the HBM budget and the
-// in-flight limit; otherwise submit returns PHMM_EBUSY.  A worker thread
-// merges every admitted request that is waiting into one batch and enqueues
-// it (copy-in, binning, forward kernels, FP64 rescue, copy-out) into one of
-// two device slots, so a new merged batch is already queued on the GPU while
-// the previous one runs; requests that arrive meanwhile join the next merge.
-// When a slot's batch completes its scores are split back per request and
-// the request's budget is released.  Every alignment is scored by a kernel
-// chosen from its own read length and coefficients only, so a request's
-// scores are the same alone and in any mix.
+// Admission (phmm_server_submit).  A request (a small host batch, usually
+// one region = one record) is admitted when its batch image fits the device
+// arena -- the HBM budget less the device memory the server's launch
+// contexts hold (engine scratch, merge buffer, scores), counted at creation
+// from the real allocations -- and the in-flight limit; otherwise submit
+// returns PHMM_EBUSY.  An admitted request is staged once through pinned
+// memory into its arena block and stays device-resident until its scores
+// are returned.
+//
+// Launches.  A worker thread merges the admitted requests that wait (in
+// arrival order, up to the per-launch caps) as soon as one of the server's
+// launch contexts is free: a gather kernel assembles the merged batch from
+// the arena on the device, then the engine scores it (binning, persistent
+// forward kernels, FP64 rescue) and the scores return to pinned memory.
+// Every context has its own stream (default priority) and its own engine
+// scratch, so a new launch is queued on the device while others run and
+// its kernels fill the compute units their tails leave idle: requests that
+// arrive while others are on the GPU are admitted, merged and overlapped.
+// A finished launch's scores are split per request, its arena blocks
+// released, and each request's result carries its own status.
+//
+// Every alignment is scored by a kernel chosen from its own read length and
+// coefficients only, so a request's scores are the same alone and in any
+// mix.
+#include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -21,16 +36,78 @@
 
 #include "common.hpp"
 #include "engine.hpp"
+#include "gather.hpp"
 
 namespace phmm {
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// First-in-first-out allocator over [0, cap) whose blocks may be released in
+// any order: a block starts after the newest one, wrapping to 0 when the end
+// is too short; released space returns once every older block is released.
+struct Ring {
+    size_t cap = 0, head = 0, live = 0;
+    struct Block {
+        size_t off, bytes;
+        bool used;
+    };
+    std::deque<Block> blocks;
+
+    bool alloc(size_t bytes, size_t* off)
+    {
+        if (bytes == 0 || bytes > cap)
+            return false;
+        size_t at;
+        if (blocks.empty()) {
+            at = 0;
+        } else {
+            const size_t tail = blocks.front().off;
+            if (head > tail) {  // free: [head, cap) and [0, tail)
+                if (cap - head >= bytes)
+                    at = head;
+                else if (tail >= bytes)
+                    at = 0;
+                else
+                    return false;
+            } else {  // wrapped: free [head, tail)
+                if (tail - head >= bytes)
+                    at = head;
+                else
+                    return false;
+            }
+        }
+        blocks.push_back({at, bytes, true});
+        head = at + bytes;
+        live += bytes;
+        *off = at;
+        return true;
+    }
+    void release(size_t off)
+    {
+        for (Block& b : blocks)
+            if (b.used && b.off == off) {
+                b.used = false;
+                live -= b.bytes;
+                break;
+            }
+        while (!blocks.empty() && !blocks.front().used)
+            blocks.pop_front();
+        if (blocks.empty())
+            head = 0;
+    }
+};
+
+constexpr size_t kAlign = 256;
+inline size_t round_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
 struct Request {
     uint64_t tag = 0;
-    HostBatch batch;
-    int64_t bytes = 0;
+    int64_t nA = 0, cells = 0;
+    int32_t G = 0, nR = 0, nH = 0;
+    int64_t rbytes = 0, hbytes = 0;
+    RequestLayout lay{};
+    size_t off = 0, bytes = 0;  // arena block
     Clock::time_point t_submit;
 };
 
@@ -38,81 +115,366 @@ struct Result {
     uint64_t tag = 0;
     std::vector<float> scores;
     double latency = 0.0;
+    int rc = PHMM_OK;
+    std::string msg;
 };
 
-// Device bytes a request occupies while admitted (staging copy, scores and
-// the engine's per-alignment work state).
-int64_t request_bytes(const phmm_batch_view& v)
-{
-    return 9 * v.read_bytes + 2 * v.hap_bytes + 24 * v.num_alignments +
-           24 * (int64_t)(v.num_reads + v.num_haps) + 64 * (int64_t)v.num_groups + 4096;
-}
+// A launch context of the server: an engine call context, its stream, the
+// merge buffer and the launch's scores.
+struct LaunchCtx {
+    int eng_ctx = -1;
+    hipStream_t stream = nullptr;
+    DevBuf buf, scores, d_reqs;
+    PinBuf pin_scores, h_reqs;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<Request> reqs;
+    int64_t n_scores = 0;
+    Clock::time_point t_launch;
+};
+
+// Groups a launch may hold per request it may merge.
+constexpr int kGroupsPerRequest = 16;
 
 }  // namespace
 }  // namespace phmm
 
 using phmm::Clock;
+using phmm::LaunchCtx;
 using phmm::Request;
 using phmm::Result;
-
 
 struct phmm_server {
     phmm_engine* handle = nullptr;
     phmm::Engine* eng = nullptr;
     std::mutex* eng_mu = nullptr;
     phmm_server_options opt{};
+    int64_t max_groups = 0;
+    // device arena of admitted requests and the pinned staging they pass
+    // through (each its own ring); the copy stream moves staged images
+    phmm::DevBuf arena;
+    phmm::Ring arena_ring;
+    phmm::PinBuf stage;
+    phmm::Ring stage_ring;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_mark = nullptr;  // recorded after every staged copy
+    std::deque<std::pair<size_t, hipEvent_t>> stage_pending;  // staging blocks in flight
+    std::vector<hipEvent_t> event_pool;
+    std::mutex submit_mu;  // staging and the copy stream, in submission order
+    int64_t fixed_bytes = 0;
+
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<Request> queue;
+    std::deque<Request> queue;  // admitted, not launched
     std::deque<Result> done;
-    struct Launch {
-        std::vector<Request> reqs;
-        int slot = 0;
-    };
-    std::deque<Launch> running;
-    phmm::Engine::Slot slots[2];
-    bool slot_busy[2] = {false, false};
+    std::vector<LaunchCtx> ctxs;
     bool stop = false;
-    int error = PHMM_OK;
-    std::string error_msg;
     phmm_server_stats st{};
     std::thread worker;
 
+    ~phmm_server();
+    int init();
     void run();
-    bool complete_front(bool block);
+    bool retire(LaunchCtx& c, bool block);
+    void launch(LaunchCtx& c, std::vector<Request>&& reqs);
+    void fail_requests(std::vector<Request>& reqs, int rc, const std::string& msg);
+    void reclaim_stage(bool block_one);
+    int check(hipError_t e, const char* what)
+    {
+        if (e == hipSuccess)
+            return PHMM_OK;
+        (void)hipGetLastError();
+        return phmm::fail(e == hipErrorOutOfMemory ? PHMM_ENOMEM : PHMM_EDEVICE,
+                          std::string("server: ") + what + ": " + hipGetErrorString(e));
+    }
 };
 
-// Retires the oldest running launch if it finished (or waits when block).
-bool phmm_server::complete_front(bool block)
+// Allocates the launch contexts at the per-launch caps (engine scratch
+// included), then the arena: the budget less what the contexts hold.
+int phmm_server::init()
 {
-    Launch& L = running.front();
-    phmm::Engine::Slot& s = slots[L.slot];
-    hipError_t q = block ? hipEventSynchronize(s.compute_done) : hipEventQuery(s.compute_done);
+    int rc;
+    std::lock_guard<std::mutex> elk(*eng_mu);
+    hipSetDevice(eng->device);
+    const int64_t A = opt.max_batch_alignments;
+    const int64_t B = opt.max_batch_bytes;
+    max_groups = (int64_t)opt.max_batch_requests * phmm::kGroupsPerRequest;
+    // a batch at every cap at once bounds the scratch of any merged launch
+    phmm_batch_view capv{};
+    capv.num_groups = (int32_t)std::min<int64_t>(max_groups, INT32_MAX);
+    capv.read_bytes = B / 5;
+    capv.hap_bytes = B;
+    capv.num_reads = (int32_t)std::min<int64_t>(B / 20, INT32_MAX);
+    capv.num_haps = (int32_t)std::min<int64_t>(B / 4, INT32_MAX);
+    capv.num_alignments = A;
+    ctxs.resize((size_t)opt.streams);
+    for (LaunchCtx& c : ctxs) {
+        c.eng_ctx = eng->claim_ctx();
+        if (c.eng_ctx < 0)
+            return phmm::fail(PHMM_EBUSY, "phmm_server_create: the engine has no free call context");
+        eng->ctxs[c.eng_ctx].spill_bytes = (size_t)32 << 20;
+        if ((rc = eng->reserve_ctx(c.eng_ctx, capv)))
+            return rc;
+        if ((rc = check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) ||
+            (rc = check(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "event")) ||
+            (rc = eng->grow(c.buf, (size_t)B, "merge buffer")) ||
+            (rc = eng->grow(c.scores, sizeof(float) * (size_t)A, "launch scores")) ||
+            (rc = eng->grow(c.d_reqs, sizeof(phmm::GatherReq) * (size_t)opt.max_batch_requests,
+                            "merge list")) ||
+            (rc = eng->ensure_pin(c.pin_scores, sizeof(float) * (size_t)A, "score staging")) ||
+            (rc = eng->ensure_pin(c.h_reqs, sizeof(phmm::GatherReq) * (size_t)opt.max_batch_requests,
+                                  "merge list staging")))
+            return rc;
+        fixed_bytes += (int64_t)(eng->ctx_bytes(c.eng_ctx) + c.buf.cap + c.scores.cap + c.d_reqs.cap);
+    }
+    if (fixed_bytes + (int64_t)(64 << 10) > opt.hbm_budget_bytes)
+        return phmm::fail(PHMM_EINVAL,
+                          "phmm_server_create: the HBM budget does not cover the launch contexts (" +
+                              std::to_string(fixed_bytes) + " bytes) and 64 KiB of requests");
+    const size_t cap = (size_t)(opt.hbm_budget_bytes - fixed_bytes) & ~(phmm::kAlign - 1);
+    if ((rc = eng->grow(arena, cap, "request arena")))
+        return rc;
+    arena_ring.cap = cap;
+    const size_t stage_cap = (size_t)std::min<int64_t>(std::max<int64_t>(2 * B, 64 << 20),
+                                                        (int64_t)cap);
+    if ((rc = eng->ensure_pin(stage, stage_cap, "request staging")))
+        return rc;
+    stage_ring.cap = stage_cap;
+    if ((rc = check(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), "stream")) ||
+        (rc = check(hipEventCreateWithFlags(&copy_mark, hipEventDisableTiming), "event")))
+        return rc;
+    st.scratch_bytes = fixed_bytes;
+    st.arena_bytes = (int64_t)cap;
+    st.bytes_in_use = fixed_bytes;
+    return PHMM_OK;
+}
+
+phmm_server::~phmm_server()
+{
+    hipSetDevice(eng->device);
+    for (LaunchCtx& c : ctxs) {
+        if (c.stream)
+            hipStreamSynchronize(c.stream);
+        for (phmm::DevBuf* b : {&c.buf, &c.scores, &c.d_reqs})
+            if (b->p)
+                hipFree(b->p);
+        for (phmm::PinBuf* b : {&c.pin_scores, &c.h_reqs})
+            if (b->p)
+                hipHostFree(b->p);
+        if (c.done)
+            hipEventDestroy(c.done);
+        if (c.stream)
+            hipStreamDestroy(c.stream);
+        if (c.eng_ctx >= 0) {
+            std::lock_guard<std::mutex> elk(*eng_mu);
+            eng->release_ctx(c.eng_ctx);
+        }
+    }
+    if (copy_stream)
+        hipStreamSynchronize(copy_stream);
+    for (auto& pr : stage_pending)
+        event_pool.push_back(pr.second);
+    for (hipEvent_t e : event_pool)
+        hipEventDestroy(e);
+    if (copy_mark)
+        hipEventDestroy(copy_mark);
+    if (copy_stream)
+        hipStreamDestroy(copy_stream);
+    if (arena.p)
+        hipFree(arena.p);
+    if (stage.p)
+        hipHostFree(stage.p);
+}
+
+// Returns staging blocks whose copies finished (waiting for the oldest one
+// when block_one).  Caller holds submit_mu.
+void phmm_server::reclaim_stage(bool block_one)
+{
+    while (!stage_pending.empty()) {
+        auto& pr = stage_pending.front();
+        const hipError_t q =
+            block_one ? hipEventSynchronize(pr.second) : hipEventQuery(pr.second);
+        if (q == hipErrorNotReady)
+            return;
+        (void)hipGetLastError();
+        stage_ring.release(pr.first);
+        event_pool.push_back(pr.second);
+        stage_pending.pop_front();
+        block_one = false;
+    }
+}
+
+void phmm_server::fail_requests(std::vector<Request>& reqs, int rc, const std::string& msg)
+{
+    const auto now = Clock::now();
+    for (Request& r : reqs) {
+        Result res;
+        res.tag = r.tag;
+        res.scores.assign((size_t)r.nA, std::nanf(""));
+        res.latency = std::chrono::duration<double>(now - r.t_submit).count();
+        res.rc = rc;
+        res.msg = msg;
+        arena_ring.release(r.off);
+        st.inflight -= 1;
+        st.completed += 1;
+        st.failed += 1;
+        done.push_back(std::move(res));
+    }
+    st.bytes_in_use = fixed_bytes + (int64_t)arena_ring.live;
+    cv.notify_all();
+}
+
+// Merges `reqs` into context c's buffer on the device and enqueues their
+// scoring; c.done fires when c.pin_scores holds the scores.
+void phmm_server::launch(LaunchCtx& c, std::vector<Request>&& reqs)
+{
+    int64_t rb = 0, hb = 0, nA = 0, cells = 0;
+    int32_t nR = 0, nH = 0, G = 0;
+    auto* gr = static_cast<phmm::GatherReq*>(c.h_reqs.p);
+    const char* abase = static_cast<const char*>(arena.p);
+    for (size_t k = 0; k < reqs.size(); ++k) {
+        const Request& r = reqs[k];
+        phmm::GatherReq& q = gr[k];
+        q.src = reinterpret_cast<const uint8_t*>(abase + r.off);
+        q.lay = r.lay;
+        q.rbytes = r.rbytes;
+        q.hbytes = r.hbytes;
+        q.rb_dst = rb;
+        q.hb_dst = hb;
+        q.nR = r.nR;
+        q.nH = r.nH;
+        q.G = r.G;
+        q.r_dst = nR;
+        q.h_dst = nH;
+        q.g_dst = G;
+        q.pad = 0;
+        rb += r.rbytes;
+        hb += r.hbytes;
+        nR += r.nR;
+        nH += r.nH;
+        G += r.G;
+        nA += r.nA;
+        cells += r.cells;
+    }
+    const phmm::RequestLayout L = phmm::request_layout(rb, hb, nR, nH, G);
+    char* D = static_cast<char*>(c.buf.p);
+    phmm::GatherDst dst{};
+    dst.read_data = reinterpret_cast<uint8_t*>(D + L.read_data);
+    dst.bq = reinterpret_cast<uint8_t*>(D + L.bq);
+    dst.iq = reinterpret_cast<uint8_t*>(D + L.iq);
+    dst.dq = reinterpret_cast<uint8_t*>(D + L.dq);
+    dst.gcp = reinterpret_cast<uint8_t*>(D + L.gcp);
+    dst.hap_data = reinterpret_cast<uint8_t*>(D + L.hap_data);
+    dst.read_len = reinterpret_cast<int32_t*>(D + L.read_len);
+    dst.hap_len = reinterpret_cast<int32_t*>(D + L.hap_len);
+    dst.read_off = reinterpret_cast<int32_t*>(D + L.read_off);
+    dst.hap_off = reinterpret_cast<int32_t*>(D + L.hap_off);
+    dst.nreads = reinterpret_cast<int32_t*>(D + L.nreads);
+    dst.nhaps = reinterpret_cast<int32_t*>(D + L.nhaps);
+    dst.rprefix = reinterpret_cast<int32_t*>(D + L.rprefix);
+    dst.hprefix = reinterpret_cast<int32_t*>(D + L.hprefix);
+    dst.read_bytes = rb;
+    dst.hap_bytes = hb;
+    dst.nR = nR;
+    dst.nH = nH;
+    dst.G = G;
+    phmm_batch_view v{};
+    v.num_groups = G;
+    v.num_reads = nR;
+    v.num_haps = nH;
+    v.read_bytes = rb;
+    v.hap_bytes = hb;
+    v.num_alignments = nA;
+    v.read_data = dst.read_data;
+    v.hap_data = dst.hap_data;
+    v.base_quals = dst.bq;
+    v.ins_quals = dst.iq;
+    v.del_quals = dst.dq;
+    v.gcp_quals = dst.gcp;
+    v.read_lengths = dst.read_len;
+    v.hap_lengths = dst.hap_len;
+    v.read_offsets = dst.read_off;
+    v.hap_offsets = dst.hap_off;
+    v.num_reads_per_group = dst.nreads;
+    v.num_haps_per_group = dst.nhaps;
+    v.reads_prefix = dst.rprefix;
+    v.haps_prefix = dst.hprefix;
+    int rc = PHMM_OK;
+    std::string msg;
+    {
+        std::lock_guard<std::mutex> elk(*eng_mu);
+        hipSetDevice(eng->device);
+        if ((size_t)L.total > c.buf.cap || (size_t)nA * sizeof(float) > c.scores.cap)
+            rc = phmm::fail(PHMM_ERANGE, "server: merged launch exceeds its buffers");
+        else if ((rc = check(hipStreamWaitEvent(c.stream, copy_mark, 0), "wait for staging")) ||
+                 (rc = check(hipMemcpyAsync(c.d_reqs.p, c.h_reqs.p,
+                                            sizeof(phmm::GatherReq) * reqs.size(),
+                                            hipMemcpyHostToDevice, c.stream),
+                             "merge list copy")) ||
+                 (rc = check(phmm::launch_gather(static_cast<const phmm::GatherReq*>(c.d_reqs.p),
+                                                 (int)reqs.size(), dst, c.stream),
+                             "gather kernel")) ||
+                 (rc = eng->score_device(v, static_cast<float*>(c.scores.p), c.stream, nullptr, 0,
+                                         c.eng_ctx)) ||
+                 (rc = check(hipMemcpyAsync(c.pin_scores.p, c.scores.p, sizeof(float) * (size_t)nA,
+                                            hipMemcpyDeviceToHost, c.stream),
+                             "score copy")))
+            ;
+        if (!rc)
+            rc = check(hipEventRecord(c.done, c.stream), "event");
+        if (rc)
+            msg = phmm::last_error();
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    st.launches += 1;
+    if (rc) {
+        fail_requests(reqs, rc, msg);
+        return;
+    }
+    st.cells += cells;
+    c.reqs = std::move(reqs);
+    c.n_scores = nA;
+    c.busy = true;
+    c.t_launch = Clock::now();
+    int64_t running = 0;
+    for (const LaunchCtx& x : ctxs)
+        running += x.busy ? 1 : 0;
+    st.max_concurrent = std::max(st.max_concurrent, running);
+}
+
+// Completes context c's launch if it finished (or waits for it when block).
+bool phmm_server::retire(LaunchCtx& c, bool block)
+{
+    if (!c.busy)
+        return false;
+    const hipError_t q = block ? hipEventSynchronize(c.done) : hipEventQuery(c.done);
     if (q == hipErrorNotReady)
         return false;
     const auto now = Clock::now();
-    std::unique_lock<std::mutex> lk(mu);
+    std::lock_guard<std::mutex> lk(mu);
+    c.busy = false;
     if (q != hipSuccess) {
-        hipGetLastError();
-        error = PHMM_EDEVICE;
-        error_msg = std::string("serving launch failed: ") + hipGetErrorString(q);
+        (void)hipGetLastError();
+        fail_requests(c.reqs, PHMM_EDEVICE,
+                      std::string("serving launch failed: ") + hipGetErrorString(q));
+        c.reqs.clear();
+        return true;
     }
-    const float* sc = static_cast<const float*>(s.pin_scores.p);
+    const float* sc = static_cast<const float*>(c.pin_scores.p);
     int64_t off = 0;
-    for (Request& r : L.reqs) {
+    for (Request& r : c.reqs) {
         Result res;
         res.tag = r.tag;
-        const int64_t n = r.batch.num_alignments;
-        res.scores.assign(sc + off, sc + off + n);
-        off += n;
+        res.scores.assign(sc + off, sc + off + r.nA);
+        off += r.nA;
         res.latency = std::chrono::duration<double>(now - r.t_submit).count();
-        st.bytes_in_use -= r.bytes;
+        arena_ring.release(r.off);
         st.inflight -= 1;
         st.completed += 1;
         done.push_back(std::move(res));
     }
-    slot_busy[L.slot] = false;
-    running.pop_front();
+    c.reqs.clear();
+    st.bytes_in_use = fixed_bytes + (int64_t)arena_ring.live;
     cv.notify_all();
     return true;
 }
@@ -120,94 +482,63 @@ bool phmm_server::complete_front(bool block)
 void phmm_server::run()
 {
     for (;;) {
+        bool progressed = false;
+        for (LaunchCtx& c : ctxs)
+            progressed |= retire(c, false);
         std::vector<Request> batch;
+        LaunchCtx* freec = nullptr;
+        bool any_busy = false;
         {
             std::unique_lock<std::mutex> lk(mu);
-            const bool can_launch = !queue.empty() && running.size() < 2;
-            if (!can_launch && running.empty()) {
-                cv.wait(lk, [&] { return stop || (!queue.empty() && running.size() < 2); });
-                if (stop && queue.empty())
-                    return;
+            for (LaunchCtx& c : ctxs) {
+                any_busy |= c.busy;
+                if (!c.busy && !freec)
+                    freec = &c;
             }
-            if (!queue.empty() && running.size() < 2) {
-                // merge waiting requests, up to the per-launch limits (at
-                // least one request), so the next launch can be staged on
-                // the host while this one computes
-                const int cap = opt.max_batch_requests > 0 ? opt.max_batch_requests : 1 << 30;
-                const int64_t acap =
-                    opt.max_batch_alignments > 0 ? opt.max_batch_alignments : INT64_MAX;
-                int64_t na = 0;
-                while (!queue.empty() && (int)batch.size() < cap &&
-                       (batch.empty() || na + queue.front().batch.num_alignments <= acap)) {
-                    na += queue.front().batch.num_alignments;
+            if (stop && queue.empty() && !any_busy)
+                return;
+            if (freec && !queue.empty()) {
+                // merge waiting requests in arrival order, up to the caps
+                int64_t na = 0, bytes = 0, groups = 0;
+                while (!queue.empty() && (int)batch.size() < opt.max_batch_requests) {
+                    const Request& r = queue.front();
+                    if (!batch.empty() && (na + r.nA > opt.max_batch_alignments ||
+                                           bytes + (int64_t)r.bytes > opt.max_batch_bytes ||
+                                           groups + r.G > max_groups))
+                        break;
+                    na += r.nA;
+                    bytes += (int64_t)r.bytes;
+                    groups += r.G;
                     batch.push_back(std::move(queue.front()));
                     queue.pop_front();
                 }
                 st.queued = (int64_t)queue.size();
+            } else if (!progressed) {
+                if (!any_busy) {
+                    cv.wait(lk, [&] { return stop || !queue.empty(); });
+                    continue;
+                }
+                if (freec) {
+                    // a context is free: wake for a new arrival or, after a
+                    // short while, to retire finished launches
+                    cv.wait_for(lk, std::chrono::microseconds(20),
+                                [&] { return stop || !queue.empty(); });
+                    continue;
+                }
             }
         }
         if (!batch.empty()) {
-            Launch L;
-            L.slot = slot_busy[0] ? 1 : 0;
-            L.reqs = std::move(batch);
-            // the requests are staged straight into the slot (no merged copy)
-            std::vector<phmm_batch_view> views;
-            std::vector<phmm::Engine::ChunkPart> parts;
-            views.reserve(L.reqs.size());
-            int64_t na = 0, cells = 0;
-            for (auto& r : L.reqs) {
-                views.push_back(r.batch.view());
-                na += r.batch.num_alignments;
-                cells += phmm_batch_cells(&views.back());
-            }
-            for (auto& v : views)
-                parts.push_back({&v, 0, v.num_groups});
-            int rc = PHMM_OK;
-            {
-                std::lock_guard<std::mutex> elk(*eng_mu);
-                hipSetDevice(eng->device);
-                if (na > 0)
-                    rc = eng->launch_parts(slots[L.slot], parts.data(), (int)parts.size());
-                else
-                    rc = eng->check(hipEventRecord(slots[L.slot].compute_done, eng->stream), "event");
-            }
-            std::unique_lock<std::mutex> lk(mu);
-            st.launches += 1;
-            if (rc == PHMM_OK)
-                st.cells += cells;
-            if (rc != PHMM_OK) {
-                error = rc;
-                error_msg = eng->err;
-                // fail the requests: return empty results so callers unblock
-                for (Request& r : L.reqs) {
-                    Result res;
-                    res.tag = r.tag;
-                    res.scores.assign((size_t)r.batch.num_alignments, __builtin_nanf(""));
-                    st.bytes_in_use -= r.bytes;
-                    st.inflight -= 1;
-                    st.completed += 1;
-                    done.push_back(std::move(res));
-                }
-                cv.notify_all();
-                continue;
-            }
-            slot_busy[L.slot] = true;
-            running.push_back(std::move(L));
+            launch(*freec, std::move(batch));
             continue;
         }
-        // nothing to launch now: retire finished work, briefly waiting for
-        // new arrivals when both slots are not busy
-        if (!running.empty()) {
-            if (complete_front(false))
-                continue;
-            std::unique_lock<std::mutex> lk(mu);
-            if (running.size() < 2 && queue.empty())
-                cv.wait_for(lk, std::chrono::microseconds(50),
-                            [&] { return stop || !queue.empty(); });
-            else if (running.size() >= 2) {
-                lk.unlock();
-                complete_front(true);
-            }
+        if (!progressed && !freec) {
+            // every context busy: nothing can launch before one finishes
+            LaunchCtx* oldest = nullptr;
+            for (LaunchCtx& c : ctxs)
+                if (c.busy && (!oldest || c.t_launch < oldest->t_launch))
+                    oldest = &c;
+            if (oldest)
+                retire(*oldest, true);
         }
     }
 }
@@ -220,7 +551,9 @@ extern "C" void phmm_server_options_default(phmm_server_options* o)
     o->hbm_budget_bytes = (int64_t)64 << 30;  // 64 GiB of request state
     o->max_inflight = 1 << 20;
     o->max_batch_requests = 4096;
-    o->max_batch_alignments = 4 << 20;
+    o->max_batch_alignments = 1 << 20;
+    o->max_batch_bytes = (int64_t)64 << 20;
+    o->streams = 2;
 }
 
 extern "C" int phmm_server_create(phmm_engine* e, const phmm_server_options* opt,
@@ -228,26 +561,31 @@ extern "C" int phmm_server_create(phmm_engine* e, const phmm_server_options* opt
 {
     if (!e || !out)
         return phmm::fail(PHMM_EINVAL, "phmm_server_create: null argument");
+    *out = nullptr;
+    phmm_server_options o;
+    if (opt)
+        o = *opt;
+    else
+        phmm_server_options_default(&o);
+    if (o.hbm_budget_bytes <= 0 || o.max_inflight <= 0 || o.max_batch_requests <= 0 ||
+        o.max_batch_alignments <= 0 || o.max_batch_alignments > INT32_MAX ||
+        o.max_batch_bytes < 4096 || o.max_batch_bytes > ((int64_t)1 << 31) || o.streams < 1 ||
+        o.streams >= phmm::Engine::kMaxCtx)
+        return phmm::fail(PHMM_EINVAL,
+                          "phmm_server_create: budget, limits and caps must be > 0, "
+                          "max_batch_bytes 4 KiB .. 2 GiB, streams 1 .. " +
+                              std::to_string(phmm::Engine::kMaxCtx - 1));
     auto* s = new (std::nothrow) phmm_server;
     if (!s)
         return phmm::fail(PHMM_ENOMEM, "phmm_server_create: out of memory");
     s->handle = e;
     s->eng = phmm::engine_of(e, &s->eng_mu);
-    if (opt)
-        s->opt = *opt;
-    else
-        phmm_server_options_default(&s->opt);
-    if (s->opt.hbm_budget_bytes <= 0 || s->opt.max_inflight <= 0) {
+    s->opt = o;
+    const int rc = s->init();
+    if (rc) {
+        const std::string m = phmm::last_error();
         delete s;
-        return phmm::fail(PHMM_EINVAL, "phmm_server_create: budget and max_inflight must be > 0");
-    }
-    for (auto& sl : s->slots) {
-        if (hipEventCreateWithFlags(&sl.h2d_done, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.compute_done, hipEventDisableTiming) != hipSuccess) {
-            hipGetLastError();
-            delete s;
-            return phmm::fail(PHMM_EDEVICE, "phmm_server_create: event creation failed");
-        }
+        return phmm::fail(rc, m);
     }
     s->worker = std::thread([s] { s->run(); });
     *out = s;
@@ -263,13 +601,9 @@ extern "C" void phmm_server_destroy(phmm_server* s)
         s->stop = true;
     }
     s->cv.notify_all();
-    // drain: the worker exits once the queue is empty; retire running work
+    // the worker launches what is queued, retires it and exits
     if (s->worker.joinable())
         s->worker.join();
-    while (!s->running.empty())
-        s->complete_front(true);
-    for (auto& sl : s->slots)
-        s->eng->free_slot(sl);
     delete s;
 }
 
@@ -280,47 +614,90 @@ extern "C" int phmm_server_submit(phmm_server* s, const phmm_batch_view* req, ui
     const std::string verr = phmm::validate_host_view(*req);
     if (!verr.empty())
         return phmm::fail(PHMM_EINVAL, "phmm_server_submit: " + verr);
-    const int64_t need = phmm::request_bytes(*req);
+    const phmm_batch_view& v = *req;
+    Request r;
+    r.tag = tag;
+    r.nA = v.num_alignments;
+    r.cells = phmm_batch_cells(&v);
+    r.G = v.num_groups;
+    r.nR = v.num_reads;
+    r.nH = v.num_haps;
+    r.rbytes = v.read_bytes;
+    r.hbytes = v.hap_bytes;
+    r.lay = phmm::request_layout(v.read_bytes, v.hap_bytes, v.num_reads, v.num_haps, v.num_groups);
+    r.bytes = phmm::round_up((size_t)r.lay.total);
+    if (r.nA > s->opt.max_batch_alignments || (int64_t)r.bytes > s->opt.max_batch_bytes ||
+        r.G > s->max_groups)
+        return phmm::fail(PHMM_ERANGE,
+                          "phmm_server_submit: request larger than one launch (alignments, "
+                          "max_batch_bytes or groups)");
+    std::lock_guard<std::mutex> sl(s->submit_mu);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop)
             return phmm::fail(PHMM_EINVAL, "phmm_server_submit: server stopping");
-        if (need > s->opt.hbm_budget_bytes)
-            return phmm::fail(PHMM_ERANGE, "phmm_server_submit: request larger than the HBM budget");
-        if (s->st.bytes_in_use + need > s->opt.hbm_budget_bytes ||
-            s->st.inflight >= s->opt.max_inflight) {
+        if (s->st.inflight >= s->opt.max_inflight || !s->arena_ring.alloc(r.bytes, &r.off)) {
             s->st.refused += 1;
-            return phmm::fail(PHMM_EBUSY, "phmm_server_submit: HBM budget or in-flight limit reached");
+            return phmm::fail(PHMM_EBUSY,
+                              "phmm_server_submit: HBM budget or in-flight limit reached");
         }
-        s->st.bytes_in_use += need;
         s->st.inflight += 1;
+        s->st.bytes_in_use = s->fixed_bytes + (int64_t)s->arena_ring.live;
     }
-    Request r;
-    r.tag = tag;
-    r.bytes = need;
+    // stage the image in pinned memory, then one copy into the arena block
+    size_t soff = 0;
+    s->reclaim_stage(false);
+    while (!s->stage_ring.alloc(r.bytes, &soff)) {
+        if (s->stage_pending.empty()) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->arena_ring.release(r.off);
+            s->st.inflight -= 1;
+            s->st.bytes_in_use = s->fixed_bytes + (int64_t)s->arena_ring.live;
+            return phmm::fail(PHMM_ERANGE, "phmm_server_submit: request larger than the staging");
+        }
+        s->reclaim_stage(true);
+    }
+    char* P = static_cast<char*>(s->stage.p) + soff;
+    const phmm::RequestLayout& L = r.lay;
+    const size_t rb = (size_t)v.read_bytes, hb = (size_t)v.hap_bytes;
+    std::memcpy(P + L.read_data, v.read_data, rb);
+    std::memcpy(P + L.bq, v.base_quals, rb);
+    std::memcpy(P + L.iq, v.ins_quals, rb);
+    std::memcpy(P + L.dq, v.del_quals, rb);
+    std::memcpy(P + L.gcp, v.gcp_quals, rb);
+    std::memcpy(P + L.hap_data, v.hap_data, hb);
+    std::memcpy(P + L.read_len, v.read_lengths, 4 * (size_t)v.num_reads);
+    std::memcpy(P + L.hap_len, v.hap_lengths, 4 * (size_t)v.num_haps);
+    std::memcpy(P + L.read_off, v.read_offsets, 4 * (size_t)v.num_reads);
+    std::memcpy(P + L.hap_off, v.hap_offsets, 4 * (size_t)v.num_haps);
+    std::memcpy(P + L.nreads, v.num_reads_per_group, 4 * (size_t)v.num_groups);
+    std::memcpy(P + L.nhaps, v.num_haps_per_group, 4 * (size_t)v.num_groups);
+    std::memcpy(P + L.rprefix, v.reads_prefix, 4 * (size_t)(v.num_groups + 1));
+    std::memcpy(P + L.hprefix, v.haps_prefix, 4 * (size_t)(v.num_groups + 1));
+    hipEvent_t ev = nullptr;
+    if (!s->event_pool.empty()) {
+        ev = s->event_pool.back();
+        s->event_pool.pop_back();
+    }
+    hipSetDevice(s->eng->device);
+    int rc = PHMM_OK;
+    if ((!ev && (rc = s->check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event"))) ||
+        (rc = s->check(hipMemcpyAsync(static_cast<char*>(s->arena.p) + r.off, P, (size_t)L.total,
+                                      hipMemcpyHostToDevice, s->copy_stream),
+                       "request copy")) ||
+        (rc = s->check(hipEventRecord(ev, s->copy_stream), "event")) ||
+        (rc = s->check(hipEventRecord(s->copy_mark, s->copy_stream), "event"))) {
+        if (ev)
+            s->event_pool.push_back(ev);
+        s->stage_ring.release(soff);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->arena_ring.release(r.off);
+        s->st.inflight -= 1;
+        s->st.bytes_in_use = s->fixed_bytes + (int64_t)s->arena_ring.live;
+        return rc;
+    }
+    s->stage_pending.push_back({soff, ev});
     r.t_submit = Clock::now();
-    // copy the request (rebased to a self-contained batch)
-    {
-        phmm::HostBatch& b = r.batch;
-        const phmm_batch_view& v = *req;
-        b.read_data.assign(v.read_data, v.read_data + v.read_bytes);
-        b.base_quals.assign(v.base_quals, v.base_quals + v.read_bytes);
-        b.ins_quals.assign(v.ins_quals, v.ins_quals + v.read_bytes);
-        b.del_quals.assign(v.del_quals, v.del_quals + v.read_bytes);
-        b.gcp_quals.assign(v.gcp_quals, v.gcp_quals + v.read_bytes);
-        b.hap_data.assign(v.hap_data, v.hap_data + v.hap_bytes);
-        b.read_lengths.assign(v.read_lengths, v.read_lengths + v.num_reads);
-        b.hap_lengths.assign(v.hap_lengths, v.hap_lengths + v.num_haps);
-        b.read_offsets.assign(v.read_offsets, v.read_offsets + v.num_reads + 1);
-        b.hap_offsets.assign(v.hap_offsets, v.hap_offsets + v.num_haps + 1);
-        b.read_offsets[(size_t)v.num_reads] = (int32_t)v.read_bytes;
-        b.hap_offsets[(size_t)v.num_haps] = (int32_t)v.hap_bytes;
-        b.num_reads.assign(v.num_reads_per_group, v.num_reads_per_group + v.num_groups);
-        b.num_haps.assign(v.num_haps_per_group, v.num_haps_per_group + v.num_groups);
-        b.reads_prefix.assign(v.reads_prefix, v.reads_prefix + v.num_groups + 1);
-        b.haps_prefix.assign(v.haps_prefix, v.haps_prefix + v.num_groups + 1);
-        b.num_alignments = v.num_alignments;
-    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->queue.push_back(std::move(r));
@@ -355,11 +732,13 @@ extern "C" int phmm_server_poll(phmm_server* s, int timeout_ms, uint64_t* tag, f
         std::memcpy(scores, r.scores.data(), sizeof(float) * r.scores.size());
     if (latency_seconds)
         *latency_seconds = r.latency;
+    // the status of this request's own launch (a failed launch fails only
+    // the requests it held)
+    const int rc = r.rc;
+    const std::string msg = r.msg;
     s->done.pop_front();
-    if (s->error != PHMM_OK) {
-        const int e = s->error;
-        return phmm::fail(e, s->error_msg);
-    }
+    if (rc != PHMM_OK)
+        return phmm::fail(rc, msg);
     return PHMM_OK;
 }
 

@@ -56,9 +56,18 @@ enum {
                           fixed at eps=0.1, beta=0.9 (pairhmm_kernels.cuh:
                           1195-1196); every non-ACGT byte is the N wildcard
                           (utility_kernels.cuh:16-20) */
-    PHMM_SEM_CPU = 1   /* reference align_host semantics: eps/beta from the
+    PHMM_SEM_CPU = 1,  /* reference align_host semantics: eps/beta from the
                           read's first gcp quality (gpuworker.cuh:1009,
-                          1048-1049) */
+                          1048-1049); only a literal 'N' is a wildcard
+                          (gpuworker.cuh:1063) */
+    PHMM_SEM_GCP = 2   /* per-position gap continuation: align_host with
+                          row r (1-based) taking eps_r = ph2pr[gcp[r-1]] and
+                          beta_r = 1 - eps_r, indexed like the row's
+                          insertion/deletion qualities (gpuworker.cuh:
+                          1059-1061); literal-'N' wildcard.  Equals
+                          PHMM_SEM_CPU when a read's gcp values are all
+                          equal.  Device views need gcp_quals 4-byte
+                          aligned. */
 };
 
 typedef struct phmm_options {
@@ -218,6 +227,12 @@ void phmm_device_batch_free(phmm_device_batch* db);
 /* Device scratch helpers for callers without their own allocator. */
 int phmm_device_alloc(phmm_engine* eng, size_t bytes, void** out);
 int phmm_device_free(phmm_engine* eng, void* ptr);
+/* Page-locked host memory (hipHostMalloc) for callers that keep their
+ * batches where the copy engines read them directly, as the reference keeps
+ * its batches in pinned memory (PinnedAllocator, common.cuh:115-116):
+ * phmm_score_batch copies such arrays to the device without staging them. */
+int phmm_host_alloc_pinned(phmm_engine* eng, size_t bytes, void** out);
+int phmm_host_free_pinned(phmm_engine* eng, void* ptr);
 int phmm_memcpy_d2h(phmm_engine* eng, void* dst, const void* src, size_t bytes);
 int phmm_memcpy_h2d(phmm_engine* eng, void* dst, const void* src, size_t bytes);
 
@@ -244,34 +259,53 @@ void phmm_free_text(char* text);
 /* Sum of readLen*hapLen over all alignments (gpuworker.cuh:495-520). */
 int64_t phmm_batch_cells(const phmm_batch_view* batch);
 
-/* ---- serving: continuous batching of active-region requests --------- */
+/* ---- serving: continuous batching of active-region requests --------- *
+ * Admitted requests stay in a device arena until their scores are returned;
+ * a worker merges waiting requests on the device (up to the per-launch caps)
+ * whenever one of `streams` launch contexts is free, so launches overlap on
+ * the GPU.  No counterpart in the reference. */
 typedef struct phmm_server_options {
-    int64_t hbm_budget_bytes;  /* device bytes of admitted request state */
+    int64_t hbm_budget_bytes;  /* device bytes the server may hold: its launch
+                                  contexts (engine scratch, merge buffers,
+                                  allocated at creation at the caps below)
+                                  plus the arena of admitted requests */
     int32_t max_inflight;      /* requests admitted but not yet returned */
     int32_t max_batch_requests;/* requests merged into one device launch */
-    int64_t max_batch_alignments; /* alignments merged into one launch (a
-                                  request larger than this runs alone) */
-    int32_t reserved[6];
+    int64_t max_batch_alignments; /* alignments per launch; a larger request
+                                  is refused with PHMM_ERANGE */
+    int64_t max_batch_bytes;   /* batch-image bytes per launch (5 x read bytes
+                                  + hap bytes + index arrays); a larger
+                                  request is refused with PHMM_ERANGE */
+    int32_t streams;           /* launch contexts (streams), 1..3; default 2 */
+    int32_t reserved[3];
 } phmm_server_options;
 
 void phmm_server_options_default(phmm_server_options* opt);
 int phmm_server_create(phmm_engine* eng, const phmm_server_options* opt, phmm_server** out);
 void phmm_server_destroy(phmm_server* srv);
 /* Submit one request (a host batch, usually one region = one group).  The
- * data is copied; PHMM_EBUSY when the budget or in-flight limit refuses it. */
+ * data is copied (the caller may reuse its buffers on return); PHMM_EBUSY
+ * when the arena (the budget) or the in-flight limit refuses it -- it is
+ * admitted again once earlier requests complete. */
 int phmm_server_submit(phmm_server* srv, const phmm_batch_view* request, uint64_t tag);
 /* Wait up to timeout_ms (< 0: forever) for a finished request; copies its
  * scores into `scores` (capacity floats) and its submit-to-completion time
  * into *latency_seconds (may be NULL).  PHMM_EAGAIN when none finished in
- * time; PHMM_ERANGE (result kept, *n_scores set) when capacity is short. */
+ * time; PHMM_ERANGE (result kept, *n_scores set) when capacity is short.  A
+ * request whose launch failed returns that launch's status (its scores are
+ * NaN); other requests are unaffected. */
 int phmm_server_poll(phmm_server* srv, int timeout_ms, uint64_t* tag, float* scores,
                      int64_t capacity, int64_t* n_scores, double* latency_seconds);
 typedef struct phmm_server_stats {
     int64_t submitted, refused, completed, launches;
     int64_t inflight, queued, max_queue_depth;
-    int64_t bytes_in_use;
+    int64_t bytes_in_use;      /* scratch_bytes + arena bytes of admitted requests */
     int64_t cells;
-    int64_t reserved[8];
+    int64_t scratch_bytes;     /* device bytes the launch contexts hold */
+    int64_t arena_bytes;       /* the request arena: budget - scratch_bytes */
+    int64_t failed;            /* requests whose launch failed */
+    int64_t max_concurrent;    /* most launches on the device at once */
+    int64_t reserved[4];
 } phmm_server_stats;
 int phmm_server_stats_get(const phmm_server* srv, phmm_server_stats* out);
 

@@ -276,46 +276,107 @@ def test_device_api_matches_host_api(engine):
     assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
 
 
-@pytest.mark.parametrize("max_batch_alignments", [None, 1])
-def test_server_alone_equals_mix(engine, max_batch_alignments):
+def _serve(srv, reqs, tags=None):
+    """Submit every request, then collect all results: {tag: scores}."""
+    tags = list(range(len(reqs))) if tags is None else tags
+    for t, r in zip(tags, reqs):
+        assert srv.submit(r, t)
+    got = {}
+    while len(got) < len(reqs):
+        res = srv.poll(timeout_ms=20000)
+        assert res is not None, "server timed out"
+        tag, scores, lat = res
+        assert lat >= 0
+        got[tag] = scores
+    return got
+
+
+@pytest.mark.parametrize("streams,max_batch_requests", [(2, 4096), (1, 4096), (2, 1), (3, 2)])
+def test_server_alone_equals_mix(engine, streams, max_batch_requests):
+    """Each request's scores are bit-identical scored alone (phmm_score_batch)
+    and served in a mix, whatever the merge and however many launches
+    overlap on the device."""
     import gpupairhmm_amd as G
-    reqs = [G.HostBatch.synthetic(4, seed=500 + i, num_records=1) for i in range(6)]
-    say(f"case: serving alone vs mix (max_batch_alignments {max_batch_alignments})")
+    reqs = [G.HostBatch.synthetic(4, seed=500 + i, num_records=1) for i in range(8)]
+    say(f"case: serving alone vs mix (streams {streams}, max_batch_requests {max_batch_requests})")
     alone = [engine.score_batch(r) for r in reqs]
-    srv = G.Server(engine, max_batch_alignments=max_batch_alignments)
+    srv = G.Server(engine, streams=streams, max_batch_requests=max_batch_requests)
     try:
-        for i, r in enumerate(reqs):
-            assert srv.submit(r, i)
-        got = {}
-        while len(got) < len(reqs):
-            res = srv.poll(timeout_ms=20000)
-            assert res is not None, "server timed out"
-            tag, scores, lat = res
-            got[tag] = scores
-            assert lat >= 0
+        got = _serve(srv, reqs)
         st = srv.stats()
     finally:
         srv.close()
     for i in range(len(reqs)):
         assert np.array_equal(alone[i].view(np.uint32), got[i].view(np.uint32))
-    assert st["completed"] == len(reqs)
-    if max_batch_alignments == 1:  # every request is its own launch
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert 1 <= st["max_concurrent"] <= streams
+    if max_batch_requests == 1:  # every request is its own launch
         assert st["launches"] == len(reqs)
 
 
-def test_server_budget_refusal(engine):
+def test_server_launches_overlap_and_merge(engine):
+    """A burst larger than one launch: requests merge, two launches are on
+    the device at once, and every score matches the request scored alone."""
     import gpupairhmm_amd as G
-    r = G.HostBatch.synthetic(4, seed=1, num_records=1)
-    srv = G.Server(engine, hbm_budget_bytes=1 << 40, max_inflight=1)
-    say("case: serving admission limit")
+    reqs = [G.HostBatch.synthetic(4, seed=600 + i, num_records=1) for i in range(40)]
+    alone = [engine.score_batch(r) for r in reqs]
+    cap = max(r.num_alignments for r in reqs) * 3
+    srv = G.Server(engine, streams=2, max_batch_alignments=cap)
+    say(f"case: serving burst of {len(reqs)} requests, {cap} alignments per launch")
     try:
+        got = _serve(srv, reqs)
+        st = srv.stats()
+    finally:
+        srv.close()
+    for i in range(len(reqs)):
+        assert np.array_equal(alone[i].view(np.uint32), got[i].view(np.uint32))
+    assert st["launches"] < len(reqs)           # merged
+    assert st["max_concurrent"] == 2            # overlapped
+    assert st["bytes_in_use"] == st["scratch_bytes"]  # every arena block released
+
+
+def test_server_budget_refusal(engine):
+    """A budget that holds the launch contexts and one request but not two:
+    the second request is refused with PHMM_EBUSY and admitted again once the
+    first one completed."""
+    import gpupairhmm_amd as G
+    # a large request (about 30k alignments of long reads and haps), so it is
+    # still on the device when the second submit follows
+    r = max((G.HostBatch.synthetic(4, seed=700 + i, num_records=1) for i in range(12)),
+            key=lambda b: b.cells())
+    caps = dict(max_batch_alignments=1 << 16, max_batch_bytes=8 << 20, max_batch_requests=8)
+    probe = G.Server(engine, **caps)
+    try:
+        fixed = probe.stats()["scratch_bytes"]
+        assert probe.submit(r, 0)
+        block = probe.stats()["bytes_in_use"] - fixed
+        assert probe.poll(timeout_ms=20000) is not None
+    finally:
+        probe.close()
+    assert block >= r.num_alignments  # the request's image is in the arena
+    budget = fixed + block + block // 2
+    srv = G.Server(engine, hbm_budget_bytes=budget, **caps)
+    say(f"case: serving budget refusal (budget {budget}, request block {block})")
+    try:
+        st = srv.stats()
+        assert st["scratch_bytes"] == fixed and st["arena_bytes"] < 2 * block
         assert srv.submit(r, 1)
-        refused = not srv.submit(r, 2)
+        assert not srv.submit(r, 2)  # PHMM_EBUSY: the arena holds one request
+        assert srv.stats()["refused"] == 1
         res = srv.poll(timeout_ms=20000)
-        assert res is not None
-        if refused:
-            assert srv.stats()["refused"] >= 1
-        assert srv.submit(r, 3)  # capacity is back after completion
-        assert srv.poll(timeout_ms=20000) is not None
+        assert res is not None and res[0] == 1
+        assert srv.submit(r, 3)  # admitted after the first completed
+        res = srv.poll(timeout_ms=20000)
+        assert res is not None and res[0] == 3
+        assert srv.stats()["bytes_in_use"] == fixed
+    finally:
+        srv.close()
+    # a request larger than one launch is refused outright
+    big = G.HostBatch.synthetic(4, seed=1, num_records=8)
+    srv = G.Server(engine, max_batch_alignments=1000)
+    try:
+        with pytest.raises(G.PairHMMError) as e:
+            srv.submit(big, 9)
+        assert e.value.code == G.ERANGE
     finally:
         srv.close()
