@@ -52,8 +52,11 @@ EXPORTED_SYMBOLS = (
     "phmm_synth_generate", "phmm_host_batch_to_text", "phmm_format_scores", "phmm_free_text",
     "phmm_batch_cells", "phmm_server_options_default", "phmm_server_create",
     "phmm_server_destroy", "phmm_server_submit", "phmm_server_poll", "phmm_server_stats_get",
-    "phmm_compute_likelihoods", "phmm_host_alloc_pinned", "phmm_host_free_pinned",
+    "phmm_compute_likelihoods", "phmm_score_device_noalloc", "phmm_host_alloc_pinned",
+    "phmm_host_free_pinned",
 )
+# Device scratch of the allocation-free entry point (PHMM_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class PairHMMError(RuntimeError):
@@ -187,6 +190,7 @@ def load_library():
         "phmm_score_batch": (ctypes.c_int, [vp, ctypes.POINTER(BatchView),
                                             ctypes.POINTER(ctypes.c_float)]),
         "phmm_score_device": (ctypes.c_int, [vp, ctypes.POINTER(BatchView), vp, vp]),
+        "phmm_score_device_noalloc": (ctypes.c_int, [vp, ctypes.POINTER(BatchView), vp, vp]),
         "phmm_compute_likelihoods": (ctypes.c_int, [vp, ctypes.POINTER(RegionRead), ctypes.c_int32,
                                                     ctypes.POINTER(RegionHap), ctypes.c_int32,
                                                     ctypes.POINTER(ctypes.c_double)]),
@@ -614,6 +618,14 @@ class Engine:
         self._chk(load_library().phmm_score_device(self._h, ctypes.byref(dbatch.view),
                                                    ctypes.c_void_p(d_scores),
                                                    ctypes.c_void_p(stream or 0)))
+
+    def score_device_noalloc(self, dbatch: DeviceBatch, d_scores: int,
+                             stream: int | None = None) -> None:
+        """phmm_score_device without any device allocation: the call's
+        scratch comes from the library's per-GPU array."""
+        self._chk(load_library().phmm_score_device_noalloc(self._h, ctypes.byref(dbatch.view),
+                                                           ctypes.c_void_p(d_scores),
+                                                           ctypes.c_void_p(stream or 0)))
 
     def synchronize(self) -> None:
         self._chk(load_library().phmm_engine_synchronize(self._h))

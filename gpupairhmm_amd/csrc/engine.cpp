@@ -1291,7 +1291,7 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")) ||
-        (rc = e.init_ctx(0)))
+        (rc = e.init_ctx(0)) || (rc = phmm::noalloc_init(device)))
         return bail(rc);
     // ph2pr[q] = (float)10^(-q/10) (gpuworker.cuh:473-486)
     float table[128];

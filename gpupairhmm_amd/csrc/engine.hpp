@@ -205,4 +205,8 @@ namespace phmm {
 // The engine behind a C handle and the mutex that serializes its users.
 Engine* engine_of(phmm_engine* h, std::mutex** mu);
 
+// Set-up of the allocation-free entry point (noalloc.hip), once per device
+// with that device current: looks up the device address of its scratch.
+int noalloc_init(int device);
+
 }  // namespace phmm

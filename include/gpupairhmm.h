@@ -215,6 +215,21 @@ int phmm_compute_likelihoods(phmm_engine* eng, const phmm_region_read* reads, in
  * launchPairHMMMultiTileKernel, gpuworker.cuh:101-125, 220-375) */
 int phmm_score_device(phmm_engine* eng, const phmm_batch_view* device_batch,
                       float* d_scores, void* hip_stream);
+/* Allocation-free variant of phmm_score_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call allocates nothing -- no hipMalloc, hipFree or
+ * allocating library routine: the batch-sized scratch comes from a device
+ * array of PHMM_NOALLOC_SCRATCH_BYTES that every GPU the library runs on
+ * holds from engine creation until the process ends.  A batch whose scratch
+ * does not fit (about 4e5 alignments and up) returns PHMM_ERANGE.  Same
+ * kernels as phmm_score_device, launched one by one and never
+ * kernel-timed: the scores are bit-identical.  Calls of one engine are
+ * ordered; calls of different engines on one GPU share that GPU's array and
+ * race if they overlap. */
+#define PHMM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int phmm_score_device_noalloc(phmm_engine* eng, const phmm_batch_view* device_batch,
+                              float* d_scores, void* hip_stream);
 /* Wait for all work the engine enqueued on its own stream. */
 int phmm_engine_synchronize(phmm_engine* eng);
 
