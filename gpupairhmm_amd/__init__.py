@@ -83,7 +83,8 @@ class Options(ctypes.Structure):
         ("semantics", ctypes.c_int32), ("fp64_rescue", ctypes.c_int32),
         ("rescue_min_sum", ctypes.c_float), ("verbose", ctypes.c_int32),
         ("time_kernels", ctypes.c_int32), ("graphs", ctypes.c_int32),
-        ("small_batch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("small_batch", ctypes.c_int32), ("f64_rows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -107,7 +108,7 @@ class Stats(ctypes.Structure):
         ("fast_alignments", ctypes.c_int64), ("multitile_alignments", ctypes.c_int64),
         ("engine_seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
         ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 6),
+        ("rescue_seconds", ctypes.c_double), ("reserved", ctypes.c_int64 * 5),
     ]
 
 
@@ -481,6 +482,7 @@ class EngineStats:
     kernel_seconds: float
     graph_launches: int = 0
     graph_captures: int = 0
+    rescue_seconds: float = 0.0
 
 
 class _PinnedBlocks:
@@ -525,7 +527,7 @@ class Engine:
     def __init__(self, device: int = 0, mode: int = MODE_BATCHED, semantics: int = SEM_GPU,
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
                  rescue_min_sum: float = FLT_MIN, time_kernels: bool = False,
-                 graphs: bool = True, small_batch: int | None = None):
+                 graphs: bool = True, small_batch: int | None = None, f64_rows: int = 0):
         lib = load_library()
         o = Options()
         lib.phmm_options_default(ctypes.byref(o))
@@ -538,6 +540,7 @@ class Engine:
         o.rescue_min_sum = float(rescue_min_sum)
         o.time_kernels = 1 if time_kernels else 0
         o.graphs = 1 if graphs else 0
+        o.f64_rows = int(f64_rows)
         h = ctypes.c_void_p()
         _check(lib.phmm_engine_create(int(device), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -671,7 +674,7 @@ class Engine:
         self._chk(load_library().phmm_engine_stats(self._h, ctypes.byref(s)))
         return EngineStats(s.alignments, s.cells, s.rescued, s.fast_alignments,
                            s.multitile_alignments, s.engine_seconds, s.kernel_seconds,
-                           s.graph_launches, s.graph_captures)
+                           s.graph_launches, s.graph_captures, s.rescue_seconds)
 
     def reset_stats(self) -> None:
         self._chk(load_library().phmm_engine_reset_stats(self._h))

@@ -21,8 +21,8 @@ namespace phmm {
 // Several counter arrays zeroed by one launch (a fill kernel rather than
 // memsets, so the sequence can be captured and replayed as a graph).
 struct ClearList {
-    int32_t* p[8];
-    int32_t n[8];
+    int32_t* p[6];
+    int32_t n[6];
     int32_t count;
 };
 __global__ void k_clear_list(ClearList cl)
@@ -943,111 +943,6 @@ __global__ void k_add_count(int64_t* total, const int32_t* counts, int n)
     }
 }
 
-// ---------------------------------------------------------------------------
-// Rescue lists ordered by hap length.  The FP32 pass appends an alignment
-// to its bin's rescue list in completion order, so the alignments the FP64
-// tile kernels take 64/G at a time have unrelated hap lengths and a wave
-// sweeps the longest of them while its other groups idle.  A counting sort
-// per bin (descending length buckets: 8 columns wide below 1024, 64 above)
-// gives every FP64 task alignments of about equal length, longest first.
-// Within a bucket the order is arbitrary: every alignment's arithmetic
-// depends on its own read and hap only.
-
-__device__ __forceinline__ int rescue_key(int hl)
-{
-    int b = hl < 1024 ? (hl >> 3) : 128 + ((hl - 1024) >> 6);
-    b = b < kRescueBuckets - 1 ? b : kRescueBuckets - 1;
-    return kRescueBuckets - 1 - b;  // longest first
-}
-
-// Bucket counts per bin (block-local counts in LDS, one global add per
-// non-empty bucket and block).
-__global__ __launch_bounds__(256) void k_rescue_hist(RescueSort rs)
-{
-    __shared__ int cnt[kRescueBuckets];
-    const BatchDesc& bd = *rs.bd;
-    for (int b = 0; b < rs.nbins; ++b) {
-        const int n = rs.counts[b];
-        if (n == 0)  // uniform: n is read by every thread
-            continue;
-        const int begin = rs.range[2 * b];
-        cnt[threadIdx.x] = 0;
-        __syncthreads();
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-            atomicAdd(&cnt[rescue_key(bd.hap_len[rs.src[begin + i].hap])], 1);
-        __syncthreads();
-        if (cnt[threadIdx.x] > 0)
-            atomicAdd(&rs.hist[b * kRescueBuckets + threadIdx.x], cnt[threadIdx.x]);
-        __syncthreads();
-    }
-}
-
-// Bucket starts per bin: the bin's list start plus the exclusive prefix of
-// its bucket counts (one block of 256 threads per bin).
-__global__ __launch_bounds__(256) void k_rescue_scan(RescueSort rs)
-{
-    __shared__ int part[kRescueBuckets];
-    const int b = blockIdx.x;
-    if (b >= rs.nbins)
-        return;
-    const int v = rs.hist[b * kRescueBuckets + threadIdx.x];
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < kRescueBuckets; o <<= 1) {
-        const int y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += y;
-        __syncthreads();
-    }
-    rs.cursor[b * kRescueBuckets + threadIdx.x] = rs.range[2 * b] + part[threadIdx.x] - v;
-}
-
-// Scatter into bucket order: positions within a block from LDS counters,
-// one global reservation per non-empty bucket and block pass.
-__global__ __launch_bounds__(256) void k_rescue_scatter(RescueSort rs)
-{
-    __shared__ int cnt[kRescueBuckets];
-    __shared__ int base[kRescueBuckets];
-    const BatchDesc& bd = *rs.bd;
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    for (int b = 0; b < rs.nbins; ++b) {
-        const int n = rs.counts[b];
-        const int begin = rs.range[2 * b];
-        // same trip count on every thread of the block (barriers inside)
-        for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-            const int i = i0 + (int)threadIdx.x;
-            WorkItem it{};
-            int key = 0, local = 0;
-            if (i < n) {
-                it = rs.src[begin + i];
-                key = rescue_key(bd.hap_len[it.hap]);
-                local = atomicAdd(&cnt[key], 1);
-            }
-            __syncthreads();
-            const int c = cnt[threadIdx.x];
-            if (c > 0) {
-                base[threadIdx.x] = atomicAdd(&rs.cursor[b * kRescueBuckets + threadIdx.x], c);
-                cnt[threadIdx.x] = 0;
-            }
-            __syncthreads();
-            if (i < n)
-                rs.dst[base[key] + local] = it;
-        }
-    }
-}
-
-hipError_t run_rescue_sort(const RescueSort& rs, bool fixed_grids, bool small, hipStream_t st)
-{
-    if (rs.nbins <= 0 || rs.nbins > kMaxBins)
-        return hipErrorInvalidValue;
-    const int grid = fixed_grids ? (small ? 128 : 1024) : 1024;
-    hipLaunchKernelGGL(k_rescue_hist, dim3(grid), dim3(256), 0, st, rs);
-    hipLaunchKernelGGL(k_rescue_scan, dim3(rs.nbins), dim3(256), 0, st, rs);
-    hipLaunchKernelGGL(k_rescue_scatter, dim3(grid), dim3(256), 0, st, rs);
-    return hipGetLastError();
-}
-
 // *total += sum of counts[0..n)
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st)
 {
@@ -1083,9 +978,7 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     cl.n[4] = kTaskCounters;
     cl.p[5] = bp.align_part;
     cl.n[5] = kAlignParts * bp.NB;
-    cl.p[6] = bp.rescue_hist;
-    cl.n[6] = kRescueBuckets * (bp.NB - 2);  // the tile bins' rescue-list buckets
-    cl.count = 7;
+    cl.count = 6;
     hipLaunchKernelGGL(k_clear_list, dim3(fixed_grids && small ? 128 : 512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.nH > 0)  // one wave per hap
         hipLaunchKernelGGL(k_hap_pass, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);

@@ -82,10 +82,9 @@ Engine::~Engine()
     if (d_ph2pr)
         hipFree(d_ph2pr);
     for (auto* v : {&ev_free, &ev_pending})
-        for (auto& pr : *v) {
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
-        }
+        for (auto& ev : *v)
+            for (hipEvent_t e : ev.e)
+                hipEventDestroy(e);
     if (stream)
         hipStreamDestroy(stream);
     if (copy_stream)
@@ -228,7 +227,6 @@ void Engine::scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[
     out[kRescueItems] = sizeof(WorkItem) * nA;
     out[kFinalItems] = sizeof(WorkItem) * nA;
     out[kHapFlag] = (size_t)std::max(v.num_haps, 1);
-    out[kRescueKeys] = sizeof(int32_t) * 2 * (size_t)kMaxBins * kRescueBuckets;
 }
 
 int Engine::ensure_pin(PinBuf& b, size_t bytes, const char* what)
@@ -250,17 +248,21 @@ int Engine::ensure_pin(PinBuf& b, size_t bytes, const char* what)
     return PHMM_OK;
 }
 
-// Adds the elapsed time of every pending event pair (waits for them).
+// Adds the elapsed times of every pending event triple (waits for them):
+// forward stage e[0] -> e[1], FP64 rescue stage e[1] -> e[2].
 int Engine::finish_timing()
 {
-    for (auto& pr : ev_pending) {
-        float ms = 0.f;
-        if (hipEventSynchronize(pr.second) == hipSuccess &&
-            hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
+    for (auto& ev : ev_pending) {
+        float ms = 0.f, ms2 = 0.f;
+        if (hipEventSynchronize(ev.e[2]) == hipSuccess &&
+            hipEventElapsedTime(&ms, ev.e[0], ev.e[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms2, ev.e[1], ev.e[2]) == hipSuccess) {
             stats.kernel_seconds += ms * 1e-3;
-        else
+            stats.rescue_seconds += ms2 * 1e-3;
+        } else {
             hipGetLastError();
-        ev_free.push_back(pr);
+        }
+        ev_free.push_back(ev);
     }
     ev_pending.clear();
     return PHMM_OK;
@@ -328,7 +330,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     static const char* const kScratchName[kNumScratch] = {
         "rows", "hap codes", "read meta", "group bins", "group bins", "read perm",
         "align offsets", "keys", "keys", "bin ranges", "work items", "alignment ranges",
-        "super-tasks", "rescue list", "final list", "hap flags", "rescue keys"};
+        "super-tasks", "rescue list", "final list", "hap flags"};
     size_t need[kNumScratch];
     scratch_sizes(v, NB, NW, need);
     void* sp[kNumScratch];
@@ -431,7 +433,6 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.rescue_counts = sm + 4;
     bp.task_ctr = sm + 128;
     bp.hapflag = static_cast<uint8_t*>(sp[kHapFlag]);
-    bp.rescue_hist = static_cast<int32_t*>(sp[kRescueKeys]);
     bp.items = static_cast<WorkItem*>(sp[kItems]);
     bp.st = static_cast<SuperTask*>(sp[kSuperTasks]);
     bp.align_range = static_cast<int32_t*>(sp[kAlignRange]);
@@ -462,12 +463,12 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     fp.semantics = opt.semantics;
 
     const bool timed = opt.time_kernels != 0 && !arena;
-    std::pair<hipEvent_t, hipEvent_t> evp{nullptr, nullptr};
+    StageEvents evp{};
     if (timed) {
         if (ev_free.empty()) {
-            if ((rc = check(hipEventCreate(&evp.first), "event")) ||
-                (rc = check(hipEventCreate(&evp.second), "event")))
-                return rc;
+            for (hipEvent_t& e : evp.e)
+                if ((rc = check(hipEventCreate(&e), "event")))
+                    return rc;
         } else {
             evp = ev_free.back();
             ev_free.pop_back();
@@ -491,13 +492,15 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
                 return rc;
         }
         if (timed)
-            hipEventRecord(evp.first, st);
+            hipEventRecord(evp.e[0], st);
         if ((rc = enqueue_fwd(fp, bp, plain, st)))
             return rc;
         if (timed)
-            hipEventRecord(evp.second, st);
+            hipEventRecord(evp.e[1], st);
         if (opt.fp64_rescue && (rc = enqueue_rescue(fp, bp, plain, st)))
             return rc;
+        if (timed)
+            hipEventRecord(evp.e[2], st);
     } else {
         // batched: replay captured graphs -- one for the whole call, or one
         // per stage when the forward kernels are timed
@@ -519,12 +522,13 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         } else {
             if ((rc = launch(kStagePrep, &graphs.prep)))
                 return rc;
-            hipEventRecord(evp.first, st);
+            hipEventRecord(evp.e[0], st);
             if ((rc = launch(kStageFwd, &graphs.fwd)))
                 return rc;
-            hipEventRecord(evp.second, st);
+            hipEventRecord(evp.e[1], st);
             if (opt.fp64_rescue && (rc = launch(kStageRescue, &graphs.rescue)))
                 return rc;
+            hipEventRecord(evp.e[2], st);
         }
     }
     if (timed)
@@ -659,29 +663,17 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     int rc;
     // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
     // the bin's form: FAST-eligible reads recompute in the FAST form).  Each
-    // bin takes the rows per lane that cost least per alignment: lanes per
-    // wave it occupies (64 / alignments per wave) times the per-step work of
-    // a lane (5 or 6 FP64 ops per row plus about 12 for the exchange).  The
-    // 10-row FAST tile (248 VGPRs) still runs two waves per SIMD; the 10-row
-    // SAFE one would not, so SAFE stops at 8.  The small-batch table takes
-    // the widest tile of each form for every bin: a small batch's FP64 time
-    // is the tail of each set launch, so one set per form beats a set per
-    // cost choice.
-    // each bin's rescue list in descending hap-length order, into the FP32
-    // pass's item list (no longer read): FP64 tasks of equal-length sweeps
-    {
-        RescueSort rs{};
-        rs.bd = bp.bd;
-        rs.src = fp.rescue_items;
-        rs.dst = bp.items;
-        rs.counts = fp.rescue_counts;
-        rs.range = bp.align_range;
-        rs.hist = bp.rescue_hist;
-        rs.cursor = bp.rescue_hist + kMaxBins * kRescueBuckets;
-        rs.nbins = 2 * ntiles;
-        if ((rc = check(run_rescue_sort(rs, !plain, act == &tiles_small, st), "rescue sort")))
-            return rc;
-    }
+    // bin takes the rows per lane nr whose modelled time per alignment is
+    // least: the tile's rows G*nr (G = lanes holding the bin's class) over
+    // the measured slot rate of full nr-row tiles (kF64SlotRate, from the
+    // autotuner), scaled by the lanes a wave leaves idle (64 - (64/G)*G) and
+    // the G fill steps of a 300-column hap.  The 10-row FAST tile (248
+    // VGPRs) still runs two waves per SIMD; the 10-row SAFE one would not,
+    // so SAFE stops at 8.  The small-batch table takes the widest tile of
+    // each form for every bin: a small batch's FP64 time is the tail of each
+    // set launch, so one set per form beats a set per cost choice.
+    // options.f64_rows forces one rows-per-lane choice (the autotuner's
+    // measurement of each).
     static constexpr int kNr64[] = {4, 6, 8, 10};
     const bool one_set = act == &tiles_small;
     // per-position gcp: the 8-row FP64 tiles in both forms
@@ -689,12 +681,12 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     BinSet set[4][2], wave;  // [rows index][SAFE, FAST]
     for (auto& row : set)
         for (BinSet& x : row) {
-            x.items = bp.items;
+            x.items = fp.rescue_items;
             x.range = bp.align_range;
             x.counts = fp.rescue_counts;
             x.count_stride = 1;
         }
-    wave.items = bp.items;
+    wave.items = fp.rescue_items;
     wave.range = bp.align_range;
     wave.counts = fp.rescue_counts;
     wave.count_stride = 1;
@@ -716,9 +708,19 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
             const int G = (tile + nr - 1) / nr;
             if (G > 64 || (!fast && nr > 8))
                 continue;
-            if (one_set && nr != (fast ? one_set_fast : 8))
+            if (opt.f64_rows > 0) {
+                if (nr != opt.f64_rows && !(!fast && opt.f64_rows > 8 && nr == 8))
+                    continue;
+            } else if (roweps() || one_set) {
+                if (nr != (fast ? one_set_fast : 8))
+                    continue;
+            }
+            const double rate = kF64SlotRate[fast ? 1 : 0][i];
+            if (!(rate > 0.0))
                 continue;
-            const double cost = (64.0 / (64 / G)) * ((fast ? 5 : 6) * nr + 12);
+            const int W = 64 / G;
+            const double cost =
+                (double)G * nr / (rate * ((double)(W * G) / 64.0) * (300.0 / (300.0 + G)));
             if (best < 0 || cost < best_cost) {
                 best = i;
                 best_cost = cost;
@@ -1142,7 +1144,9 @@ int Engine::score_batch(const phmm_batch_view& v, float* out)
             int64_t na = 0;
             for (int g = ch.first; g < ch.second; ++g)
                 na += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
-            const int n = na >= ((int64_t)1 << 19) ? 4 : na >= ((int64_t)1 << 17) ? 2 : 1;
+            // pieces of at least 3e5 alignments keep the tuned tile table
+            // (calls of up to options.small_batch use the one-template one)
+            const int n = (int)std::max<int64_t>(1, std::min<int64_t>(4, na / 300000));
             int g0 = ch.first;
             int64_t acc = 0;
             int k = 1;
@@ -1257,6 +1261,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown mode");
     if (o.semantics != PHMM_SEM_GPU && o.semantics != PHMM_SEM_CPU && o.semantics != PHMM_SEM_GCP)
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown semantics");
+    if (o.f64_rows != 0 && o.f64_rows != 4 && o.f64_rows != 6 && o.f64_rows != 8 && o.f64_rows != 10)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: f64_rows must be 0, 4, 6, 8 or 10");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         hipGetLastError();

@@ -48,7 +48,7 @@ struct Engine {
     enum ScratchBuf {
         kRows, kHapc, kMeta, kGbCount, kGbStart, kReadPerm, kAlignOff, kKeyCount, kKeyCursor,
         kBinRange, kItems, kAlignRange, kSuperTasks, kRescueItems, kFinalItems, kHapFlag,
-        kRescueKeys, kNumScratch
+        kNumScratch
     };
     // An arena must leave at least this much for the spill arena (one block
     // of the FP64 wave kernel holds 32 bytes per column of the longest hap).
@@ -112,8 +112,11 @@ struct Engine {
     }
     CallCtx* cur = &ctxs[0];  // the context of the call being enqueued
     int32_t* h_small = nullptr;       // pinned, small readbacks
-    // event pairs around the forward kernels of timed calls
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_pending;
+    // events around the forward and rescue stages of timed calls
+    struct StageEvents {
+        hipEvent_t e[3];  // before the forward stage, after it, after the rescue
+    };
+    std::vector<StageEvents> ev_free, ev_pending;
 
     int64_t* d_rescued_total = nullptr;  // cumulative FP64 rescues
 

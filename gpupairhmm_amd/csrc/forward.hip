@@ -132,6 +132,25 @@ struct TileTraits<double> {
     __device__ static double shr(double x) { return shr1d(x); }
 };
 
+// Shape of a lane's emission table in LDS: per letter, NV vectors of K rows
+// (the last may hold fewer), tab[(letter*NV + q)*64 + lane] -- one ds_read
+// of 16 bytes per vector, conflict-free because consecutive lanes read
+// consecutive 16-byte words.  A last vector holding a single row would be
+// read as one 4-byte word at a 16-byte lane stride, four lanes to a bank,
+// so such a row (kTail) sits in a table of scalars instead,
+// stab[letter*64 + lane], at a 4-byte lane stride.
+template <typename T, int NR>
+struct TabShape {
+    static constexpr int K = TileTraits<T>::K;
+    static constexpr bool kTail = K > 1 && NR % K == 1;
+    static constexpr int NV = kTail ? NR / K : (NR + K - 1) / K;
+    static constexpr int kVecs = 4 * NV * 64;
+    static constexpr int kWords =  // table size in units of the vector type
+        kVecs + (kTail ? (4 * 64 * (int)sizeof(T) + (int)sizeof(typename TileTraits<T>::V) - 1) /
+                             (int)sizeof(typename TileTraits<T>::V)
+                       : 0);
+};
+
 // Per-lane state of a tile task that depends only on the read: its rows'
 // coefficients (the emission values of the rows live in the LDS table).
 // ROWEPS: every row has its own gap continuation eps_r (per-position gcp
@@ -188,7 +207,6 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
     constexpr int K = TT::K;
-    constexpr int NQ = (NR + K - 1) / K;  // the last vector may hold fewer than K rows
     const int lane = threadIdx.x;
     int L = 0, roff = 0;
     float nl = 1.0f;
@@ -287,13 +305,16 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
                 }
             }
         }
-        // emission table: tab[(letter*NQ + q)*64 + lane] = rows K*q .. K*q+K-1
+        // emission table: tab[(letter*NV + q)*64 + lane] = rows K*q .. K*q+K-1
         // for the four bases (haps with any other byte go to the one-wave
-        // kernel; a read's 'N' matches every base)
+        // kernel; a read's 'N' matches every base), a single last row in the
+        // scalar table
+        using Shape = TabShape<T, NR>;
+        constexpr int NV = Shape::NV;
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+            for (int q = 0; q < NV; ++q) {
                 T v[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -305,7 +326,12 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
                         v[k] = 0;  // past the lane's rows: never used
                     }
                 }
-                tab[(l * NQ + q) * 64 + lane] = TT::pack(v);
+                tab[(l * NV + q) * 64 + lane] = TT::pack(v);
+            }
+            if constexpr (Shape::kTail) {
+                constexpr int j = NR - 1;
+                const bool m = (l == code[j]) || (code[j] == 4);
+                reinterpret_cast<T*>(tab + Shape::kVecs)[l * 64 + lane] = m ? lm[j] : lx[j];
             }
         }
     }
@@ -329,7 +355,6 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     using TT = TileTraits<T>;
     using V = typename TT::V;
     constexpr int K = TT::K;
-    constexpr int NQ = (NR + K - 1) / K;  // the last vector may hold fewer than K rows
     constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x;
     const bool first = (t == 0);
@@ -388,13 +413,19 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
 
     // emission values of step 0; each step loads the next step's values
     // before it computes
-    V lam[2][NQ];
+    using Shape = TabShape<T, NR>;
+    constexpr int NV = Shape::NV;
+    const T* stab = reinterpret_cast<const T*>(tab + Shape::kVecs) + lane_off;
+    V lam[2][NV];
+    T lamS[2] = {(T)0, (T)0};  // the scalar-table row (Shape::kTail)
     uint32_t letters = __builtin_amdgcn_alignbyte(whi, wlo, shift);
     {
-        const V* trow = tab + __umul24(letters & 3u, NQ * 64) + lane_off;
+        const V* trow = tab + __umul24(letters & 3u, NV * 64) + lane_off;
 #pragma unroll
-        for (int qq = 0; qq < NQ; ++qq)
+        for (int qq = 0; qq < NV; ++qq)
             lam[0][qq] = trow[qq * 64];
+        if constexpr (Shape::kTail)
+            lamS[0] = stab[__umul24(letters & 3u, 64)];
     }
 
     auto sweep4 = [&](int s4, bool bounded) {
@@ -407,38 +438,46 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
             // reads as base 0 there, and padding columns are never accumulated
             const uint32_t nletter = u < 3 ? (letters >> (8 * (u + 1))) & 3u : next & 3u;
             {
-                const V* trow = tab + __umul24(nletter, NQ * 64) + lane_off;
+                const V* trow = tab + __umul24(nletter, NV * 64) + lane_off;
 #pragma unroll
-                for (int qq = 0; qq < NQ; ++qq)
+                for (int qq = 0; qq < NV; ++qq)
                     lam[nxt][qq] = trow[qq * 64];
+                if constexpr (Shape::kTail)
+                    lamS[nxt] = stab[__umul24(nletter, 64)];
             }
             T mp = Mr[cur], dp = Dr[cur], chain = Iin;
+            // the cell of row j with emission value lvk
+            auto cell = [&](int j, T lvk) {
+                const T mo = M[j], dov = D[j];
+                T t1;
+                if (FAST) {
+                    t1 = fma(R.c1[j], dp, mp);
+                    t1 = fma(R.c0[j], chain, t1);
+                    chain = fma(R.c2[j], chain, mp);
+                } else {
+                    t1 = fma(R.c1[j], dp, chain);
+                    t1 = fma(R.c0[j], mp, t1);
+                    chain = fma(R.c2[j], mp, R.c3[j] * chain);
+                }
+                M[j] = lvk * t1;
+                D[j] = fma((d_eps<T, NR, FAST, ROWEPS>(R, j)), dov, mo);
+                mp = mo;
+                dp = dov;
+            };
 #pragma unroll
-            for (int qq = 0; qq < NQ; ++qq) {
+            for (int qq = 0; qq < NV; ++qq) {
                 T lv[K];
                 TT::unpack(lam[cur][qq], lv);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const int j = K * qq + k;
-                    if (j >= NR)
+                    if (j >= (Shape::kTail ? NR - 1 : NR))
                         break;
-                    const T mo = M[j], dov = D[j];
-                    T t1;
-                    if (FAST) {
-                        t1 = fma(R.c1[j], dp, mp);
-                        t1 = fma(R.c0[j], chain, t1);
-                        chain = fma(R.c2[j], chain, mp);
-                    } else {
-                        t1 = fma(R.c1[j], dp, chain);
-                        t1 = fma(R.c0[j], mp, t1);
-                        chain = fma(R.c2[j], mp, R.c3[j] * chain);
-                    }
-                    M[j] = lv[k] * t1;
-                    D[j] = fma((d_eps<T, NR, FAST, ROWEPS>(R, j)), dov, mo);
-                    mp = mo;
-                    dp = dov;
+                    cell(j, lv[k]);
                 }
             }
+            if constexpr (Shape::kTail)
+                cell(NR - 1, lamS[cur]);
             if (bounded)
                 acc += (s4 + u <= acc_last) ? chain : (T)0;
             else
@@ -606,12 +645,23 @@ __device__ __forceinline__ int next_task(int32_t* ctr)
 // Bins of one template in one persistent launch.  Waves take tasks in order
 // from a counter: consecutive tasks of a length-sorted bin have similar
 // lengths, and a wave that finishes early takes the next one.
+// Waves per SIMD a tile kernel is built for: the 12-row tiles and the
+// 13-row FAST tile fit three (at most 168 VGPRs, no spills) and issue more
+// of their dependent FMA chains per cycle that way; the larger tiles need
+// up to 256 VGPRs and run two.
+template <typename T, int NR, bool FAST>
+constexpr int tile_waves_per_simd()
+{
+    return (sizeof(T) == 4 && (NR == 12 || (NR == 13 && FAST))) ? 3 : 1;
+}
+
 template <typename T, int NR, bool FAST, bool ROWEPS = false>
-__global__ __launch_bounds__(64) void k_fwd_tile_set(FwdParams p, BinSet bs)
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST>(), 8))) void k_fwd_tile_set(
+    FwdParams p, BinSet bs)
 {
     static_assert(NR >= 4 && NR <= 32, "rows per lane");
-    constexpr int NQ = (NR + TileTraits<T>::K - 1) / TileTraits<T>::K;
-    __shared__ typename TileTraits<T>::V tab[4 * NQ * 64];
+    __shared__ typename TileTraits<T>::V tab[TabShape<T, NR>::kWords];
     SetLane sl;
     const int total = set_lanes(bs, 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic

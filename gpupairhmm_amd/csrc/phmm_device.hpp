@@ -69,7 +69,6 @@ constexpr int kKeyParts = 8;        // counters per (bin, bucket): groups spread
 constexpr int kKeysPerBin = kHapBuckets * kKeyParts;
 constexpr int kAlignParts = 256;    // partial counters per bin (atomic spreading)
 constexpr int kHmaxParts = 512;     // partial maxima of the hap pass (atomic spreading)
-constexpr int kRescueBuckets = 256; // hap-length buckets of the rescue-list sort
 
 // Hap length -> sort bucket: exact below 2048, then width 16.
 __host__ __device__ inline int hap_bucket(int h)
@@ -185,7 +184,6 @@ struct BinParams {
     int32_t* align_range;      // [2*NB]: per bin, begin / count in alignments
     int32_t* align_part;       // [NB*kAlignParts]: partial alignment counts per bin (k_scan_keys sums each bin's into its first)
     uint8_t* hapflag;          // [nH]: 1 = hap holds a byte other than A/C/G/T
-    int32_t* rescue_hist;      // [2*kMaxBins*kRescueBuckets]: rescue-list bucket counts, then cursors
     WorkItem* items;
     const float* ph2pr;
     // tiles
@@ -203,22 +201,6 @@ struct BinParams {
 hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, bool small,
                        hipStream_t st);
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
-
-// Counting sort of the tile bins' rescue lists (bins 0 .. nbins-1, list of
-// bin b at src + range[2b], counts[b] long) into dst at the same offsets,
-// each list ordered by descending hap-length bucket.  hist (zeroed before)
-// and cursor hold nbins * kRescueBuckets counters.
-struct RescueSort {
-    const BatchDesc* bd;
-    const WorkItem* src;
-    WorkItem* dst;
-    const int32_t* counts;
-    const int32_t* range;
-    int32_t* hist;
-    int32_t* cursor;
-    int32_t nbins;
-};
-hipError_t run_rescue_sort(const RescueSort& rs, bool fixed_grids, bool small, hipStream_t st);
 // Several bins in one persistent launch.  Bin bin[i] holds count_i =
 // counts[count_stride*bin[i]] units starting at begin_i = range ?
 // range[2*bin[i]] : 0, both read on the device.  A unit is a SuperTask

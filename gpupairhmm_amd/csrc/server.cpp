@@ -33,6 +33,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <vector>
 
 #include "common.hpp"
 #include "engine.hpp"
@@ -174,6 +175,7 @@ struct phmm_server {
 
     ~phmm_server();
     int init();
+    int create_launch_stream(hipStream_t* out);
     void run();
     bool retire(LaunchCtx& c, bool block);
     void launch(LaunchCtx& c, std::vector<Request>&& reqs);
@@ -188,6 +190,23 @@ struct phmm_server {
                           std::string("server: ") + what + ": " + hipGetErrorString(e));
     }
 };
+
+// A launch context's stream, on a hardware queue of its own: streams
+// otherwise share the process's few queues, where kernels of different
+// streams run one after another and launches could not overlap.  A stream
+// created with a CU mask gets its own queue; the mask names every CU.
+// Default priority, like every stream here.
+int phmm_server::create_launch_stream(hipStream_t* out)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, eng->device) == hipSuccess && prop.multiProcessorCount > 0) {
+        std::vector<uint32_t> mask((size_t)(prop.multiProcessorCount + 31) / 32, 0xffffffffu);
+        if (hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data()) == hipSuccess)
+            return PHMM_OK;
+    }
+    (void)hipGetLastError();
+    return check(hipStreamCreateWithFlags(out, hipStreamNonBlocking), "stream");
+}
 
 // Allocates the launch contexts at the per-launch caps (engine scratch
 // included), then the arena: the budget less what the contexts hold.
@@ -215,7 +234,7 @@ int phmm_server::init()
         eng->ctxs[c.eng_ctx].spill_bytes = (size_t)32 << 20;
         if ((rc = eng->reserve_ctx(c.eng_ctx, capv)))
             return rc;
-        if ((rc = check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) ||
+        if ((rc = create_launch_stream(&c.stream)) ||
             (rc = check(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "event")) ||
             (rc = eng->grow(c.buf, (size_t)B, "merge buffer")) ||
             (rc = eng->grow(c.scores, sizeof(float) * (size_t)A, "launch scores")) ||

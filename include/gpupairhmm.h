@@ -94,7 +94,11 @@ typedef struct phmm_options {
                                   rather than one per tile template; results
                                   are bit-identical.  0: always the tuned
                                   table.  Default 1 << 18. */
-    int32_t reserved[4];
+    int32_t f64_rows;          /* rows per lane of every FP64 rescue tile
+                                  (4, 6, 8 or 10; SAFE tiles at most 8); 0
+                                  (default): chosen per bin from the
+                                  measured FP64 tile rates */
+    int32_t reserved[3];
 } phmm_options;
 
 /* ---- batch view (BatchOfSequenceGroups, common.cuh:109-201) --------- *
@@ -147,7 +151,8 @@ typedef struct phmm_stats {
     int64_t graph_launches;    /* HIP graph replays (batched mode) */
     int64_t graph_captures;    /* graphs captured (first call, new scratch
                                   sizes or a new tile table) */
-    int64_t reserved[6];
+    double rescue_seconds;     /* event-timed FP64 rescue stage (if timed) */
+    int64_t reserved[5];
 } phmm_stats;
 
 /* One row of the tile table (the reference's per-architecture

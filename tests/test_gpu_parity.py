@@ -42,17 +42,97 @@ def test_readme_known_answer(engine, oracle):
     assert text.splitlines()[0] == "2 2"
 
 
-@pytest.mark.parametrize("config,records", [(0, 1000), (1, 40), (2, 3), (3, 4), (4, 3)])
-def test_config_parity(engine, oracle, config, records):
+@pytest.mark.parametrize("config,records,sample",
+                         [(0, 1000, 0), (1, 40, 0), (2, 60, 40000), (3, 20, 0), (4, 6, 0)])
+def test_config_parity(engine, oracle, config, records, sample):
+    """Every configuration against the FP64 oracle (all alignments, or a
+    seeded sample of `sample` alignments of a larger batch)."""
     import gpupairhmm_amd as G
     b = G.HostBatch.synthetic(config, seed=1000 + config, num_records=records)
     say(f"case: config {config}, {records} records, {b.num_alignments} alignments")
     got = engine.score_batch(b)
-    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
-    d = _max_delta(got, ref)
-    print(f"config {config}: max |delta log10| = {d:.3e}")
+    if sample and sample < b.num_alignments:
+        which = np.sort(np.random.default_rng(config).choice(b.num_alignments, sample,
+                                                             replace=False))
+        ref = oracle.score_subset(b.oracle_arrays(), which, oracle.F64)
+        got_s = got[which]
+    else:
+        ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+        got_s = got
+    d = _max_delta(got_s, ref)
+    print(f"config {config}: max |delta log10| = {d:.3e} over {len(ref)} alignments")
     assert np.all(np.isfinite(got))
     assert d <= TOL
+
+
+def test_tuned_table_batched_parity(lib, oracle):
+    """A batch above the small-batch size (so the tuned table's persistent
+    multi-bin launches, multi-hap segments and cross-record item bins run)
+    against the FP64 oracle on a seeded sample."""
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(2, seed=2020, num_records=300)
+    assert b.num_alignments > (1 << 18)
+    say(f"case: tuned table, 300 config-2 records, {b.num_alignments} alignments")
+    with G.Engine(small_batch=0) as e:
+        got = e.score_batch(b)
+    which = np.sort(np.random.default_rng(7).choice(b.num_alignments, 20000, replace=False))
+    ref = oracle.score_subset(b.oracle_arrays(), which, oracle.F64)
+    d = _max_delta(got[which], ref)
+    print(f"tuned table: max |delta log10| = {d:.3e}")
+    assert np.all(np.isfinite(got)) and d <= TOL
+
+
+def _every_class_batch(tiles, seed=31):
+    """One record per tile class: reads at the class's bound and just above
+    the previous bound, half of them with constant insertion/deletion
+    qualities (the rescaled FAST form admits them) and half with qualities
+    that jump between rows (e_r > 1/2: the SAFE form), against haps of
+    several lengths; plus a read beyond the largest tile."""
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(seed)
+    lines, prev = [], 0
+    bounds = [t[2] for t in tiles] + [tiles[-1][2] + 200]
+    for b in bounds:
+        reads = []
+        for L in sorted({max(1, prev + 1), b}):
+            for form in ("fast", "safe"):
+                r = "".join(rng.choice(list("ACGT"), L))
+                q = "".join(chr(33 + int(x)) for x in rng.integers(10, 41, L))
+                if form == "fast":
+                    gi = "I" * (L - 1) + "N" if L > 1 else "N"
+                else:
+                    gi = "".join(chr(33 + int(x)) for x in rng.integers(10, 46, L))
+                reads.append(" ".join([r, q, gi, gi, "+" * L]))
+        haps = ["".join(rng.choice(list("ACGT"), int(H))) for H in rng.integers(30, 400, 3)]
+        lines.append(f"{len(reads)} {len(haps)}")
+        lines += reads + haps
+        prev = b
+    return G.HostBatch.parse("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("small_batch", [None, 0])
+def test_every_tile_class_and_form_in_one_call(lib, oracle, small_batch):
+    """Reads of every tile class, in both arithmetic forms, in one batched
+    call (the one-template table and the tuned table), against the oracle;
+    the plain engine gives the same bits."""
+    import gpupairhmm_amd as G
+    with G.Engine(small_batch=small_batch) as e:
+        b = _every_class_batch(e.tiles())
+        say(f"case: every tile class and form, small_batch={small_batch}, "
+            f"{b.num_alignments} alignments")
+        got = e.score_batch(b)
+        st = e.stats()
+    with G.Engine(mode=G.MODE_PLAIN) as p:
+        plain = p.score_batch(b)
+        pst = p.stats()
+    ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
+    assert _max_delta(got, ref) <= TOL
+    assert np.array_equal(got.view(np.uint32), plain.view(np.uint32))
+    # both forms ran: some alignments FAST, some not; the long read went to
+    # the multi-tile kernel
+    assert 0 < pst.fast_alignments < b.num_alignments
+    assert pst.multitile_alignments >= 3
+    del st
 
 
 def test_plain_and_batched_identical(engine, plain_engine):
