@@ -190,11 +190,18 @@ def workload_lengths(max_len=1023):
     return p / p.sum()
 
 
+def class_floor(b: int) -> int:
+    """Smallest lower bound a of a class (a, b]: above 31 rows a class holds
+    no read shorter than two thirds of its bound, so a read pays for at most
+    half as many rows again as it has, whatever the workload's weights."""
+    return 0 if b <= 31 else (2 * b) // 3
+
+
 def choose_bounds(rates, p, k=MAX_CLASSES, fast_share=FAST_SHARE, rows=ROWS):
     """Class bounds b_1 < ... < b_k = len(p)-1 minimising
     sum_L p(L) * cost(tile of L's class), where a class (a, b] takes its
-    cheapest modelled tile holding b + 1 rows.  Dynamic programming over the
-    bounds; returns [(bound, lanes, rows)]."""
+    cheapest modelled tile holding b + 1 rows and a >= class_floor(b).
+    Dynamic programming over the bounds; returns [(bound, lanes, rows)]."""
     n = len(p) - 1
     P = np.concatenate([[0.0], np.cumsum(p[1:])])  # P[b] = mass of lengths 1..b
     best_tile = [None] * (n + 1)
@@ -214,10 +221,10 @@ def choose_bounds(rates, p, k=MAX_CLASSES, fast_share=FAST_SHARE, rows=ROWS):
         for b in range(1, n + 1):
             if not np.isfinite(cost[b]):
                 continue
-            a = np.arange(b)
-            tot = prev[:b] + (P[b] - P[a]) * cost[b]
+            a = np.arange(class_floor(b), b)
+            tot = prev[a] + (P[b] - P[a]) * cost[b]
             i = int(np.argmin(tot))
-            dp[j][b], arg[j][b] = tot[i], i
+            dp[j][b], arg[j][b] = tot[i], a[i]
     j = int(np.argmin(dp[:, n]))
     out, b = [], n
     while b > 0 and j > 0:
@@ -251,8 +258,6 @@ def measure_f64(G, cells, reps, log=print):
     for form in ("fast", "safe"):
         batch = make_batch(G, 151, 300, cells / 8, seed=64, safe=form == "safe")
         ncell = batch.cells()
-        eng.free(d)
-        db.free()
         for nr in ROWS64:
             if form == "safe" and nr > 8:
                 continue  # 10-row SAFE FP64 tiles need more than 256 VGPRs
@@ -313,7 +318,7 @@ def main(argv=None):
         meas = measure_classes(G, eng, probe, hap_lens, cells, args.reps)
         rates = fit_rates(meas)
         print("slot rates (Gcells/s):",
-              {f"{r},{f}": round(v / 1e9, 1) for (r, f), v in sorted(rates.items())}, flush=True)
+              {f"{r},{f}": round(v, 1) for (r, f), v in sorted(rates.items())}, flush=True)
         p = workload_lengths()
         bounds, model = choose_bounds(rates, p, k=args.classes)
         print("model bounds:", bounds, flush=True)

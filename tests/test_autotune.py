@@ -34,6 +34,9 @@ def test_choose_bounds_covers_every_length_with_fitting_tiles():
     assert bounds == sorted(set(bounds)) and bounds[-1] == 1023 and len(table) <= 12
     for b, g, r in table:
         assert g * r >= b + 1 and 1 <= g <= 64
+    # no class spans lengths below its floor, whatever the weights
+    for (a, _, _), (b, _, _) in zip([(0, 0, 0)] + table[:-1], table):
+        assert a >= A.class_floor(b)
     # the chosen table is no worse under the model than any fixed table,
     # e.g. the probe bounds with their cheapest tiles
     probe = []
