@@ -84,7 +84,7 @@ class Options(ctypes.Structure):
         ("rescue_min_sum", ctypes.c_float), ("verbose", ctypes.c_int32),
         ("time_kernels", ctypes.c_int32), ("graphs", ctypes.c_int32),
         ("small_batch", ctypes.c_int32), ("f64_rows", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("range_rescue", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -108,7 +108,8 @@ class Stats(ctypes.Structure):
         ("fast_alignments", ctypes.c_int64), ("multitile_alignments", ctypes.c_int64),
         ("engine_seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
         ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
-        ("rescue_seconds", ctypes.c_double), ("reserved", ctypes.c_int64 * 5),
+        ("rescue_seconds", ctypes.c_double), ("f64_alignments", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 4),
     ]
 
 
@@ -483,6 +484,7 @@ class EngineStats:
     graph_launches: int = 0
     graph_captures: int = 0
     rescue_seconds: float = 0.0
+    f64_alignments: int = 0
 
 
 class _PinnedBlocks:
@@ -527,7 +529,8 @@ class Engine:
     def __init__(self, device: int = 0, mode: int = MODE_BATCHED, semantics: int = SEM_GPU,
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
                  rescue_min_sum: float = FLT_MIN, time_kernels: bool = False,
-                 graphs: bool = True, small_batch: int | None = None, f64_rows: int = 0):
+                 graphs: bool = True, small_batch: int | None = None, f64_rows: int = 0,
+                 range_rescue: bool = True):
         lib = load_library()
         o = Options()
         lib.phmm_options_default(ctypes.byref(o))
@@ -541,6 +544,7 @@ class Engine:
         o.time_kernels = 1 if time_kernels else 0
         o.graphs = 1 if graphs else 0
         o.f64_rows = int(f64_rows)
+        o.range_rescue = 1 if range_rescue else 0
         h = ctypes.c_void_p()
         _check(lib.phmm_engine_create(int(device), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -674,7 +678,8 @@ class Engine:
         self._chk(load_library().phmm_engine_stats(self._h, ctypes.byref(s)))
         return EngineStats(s.alignments, s.cells, s.rescued, s.fast_alignments,
                            s.multitile_alignments, s.engine_seconds, s.kernel_seconds,
-                           s.graph_launches, s.graph_captures, s.rescue_seconds)
+                           s.graph_launches, s.graph_captures, s.rescue_seconds,
+                           s.f64_alignments)
 
     def reset_stats(self) -> None:
         self._chk(load_library().phmm_engine_reset_stats(self._h))

@@ -21,8 +21,8 @@ namespace phmm {
 // Several counter arrays zeroed by one launch (a fill kernel rather than
 // memsets, so the sequence can be captured and replayed as a graph).
 struct ClearList {
-    int32_t* p[6];
-    int32_t n[6];
+    int32_t* p[7];
+    int32_t n[7];
     int32_t count;
 };
 __global__ void k_clear_list(ClearList cl)
@@ -978,7 +978,9 @@ hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grid
     cl.n[4] = kTaskCounters;
     cl.p[5] = bp.align_part;
     cl.n[5] = kAlignParts * bp.NB;
-    cl.count = 6;
+    cl.p[6] = bp.retry_counts;
+    cl.n[6] = kRetrySlots;
+    cl.count = 7;
     hipLaunchKernelGGL(k_clear_list, dim3(fixed_grids && small ? 128 : 512), dim3(tpb), 0, st, cl);
     if (fixed_grids || hd.nH > 0)  // one wave per hap
         hipLaunchKernelGGL(k_hap_pass, dim3(grid(hd.nH, tpb / 64, 4096)), dim3(tpb), 0, st, bp);

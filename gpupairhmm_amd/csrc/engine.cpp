@@ -151,7 +151,8 @@ int Engine::init_ctx(int k)
     if (c.ready)
         return PHMM_OK;
     int rc;
-    if ((rc = ensure(c, c.smallints, sizeof(int32_t) * (256 + kHmaxParts), "counters")) ||
+    if ((rc = ensure(c, c.smallints, sizeof(int32_t) * (256 + kHmaxParts + kRetrySlots),
+                     "counters")) ||
         (rc = check(hipMemset(c.smallints.p, 0, c.smallints.cap), "memset")) ||
         (rc = ensure(c, c.desc, sizeof(BatchDesc), "batch descriptor")) ||
         (rc = check(hipHostMalloc(reinterpret_cast<void**>(&c.h_desc),
@@ -174,8 +175,8 @@ int Engine::init_ctx(int k)
 void Engine::bins_for(const phmm_batch_view& v, int* NB, int* NW) const
 {
     // the table a call of v uses (see score_device)
-    const bool small = roweps() || (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 &&
-                                    v.num_alignments <= opt.small_batch);
+    const bool small =
+        roweps() || (opt.small_batch > 0 && v.num_alignments <= opt.small_batch);
     const int ntiles = (int)(small ? tiles_small : tiles).size();
     *NB = 2 * ntiles + 2;
     *NW = *NB + 2 * ntiles;
@@ -227,6 +228,8 @@ void Engine::scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[
     out[kRescueItems] = sizeof(WorkItem) * nA;
     out[kFinalItems] = sizeof(WorkItem) * nA;
     out[kHapFlag] = (size_t)std::max(v.num_haps, 1);
+    // per-lane magnitudes of rescued alignments: 16 bytes per rescue-list slot
+    out[kRescueExps] = (size_t)kReKnownLanes * nA;
 }
 
 int Engine::ensure_pin(PinBuf& b, size_t bytes, const char* what)
@@ -312,8 +315,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     // keeps the per-class launches of the tuned table); per-position gcp
     // always does: its tile kernels (a gap continuation register per row)
     // exist for 16 rows per lane
-    act = (roweps() || (opt.mode == PHMM_MODE_BATCHED && opt.small_batch > 0 &&
-                        v.num_alignments <= opt.small_batch))
+    act = (roweps() || (opt.small_batch > 0 && v.num_alignments <= opt.small_batch))
               ? &tiles_small
               : &tiles;
     const std::vector<phmm_tile_config>& tiles = *act;
@@ -330,7 +332,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     static const char* const kScratchName[kNumScratch] = {
         "rows", "hap codes", "read meta", "group bins", "group bins", "read perm",
         "align offsets", "keys", "keys", "bin ranges", "work items", "alignment ranges",
-        "super-tasks", "rescue list", "final list", "hap flags"};
+        "super-tasks", "rescue list", "final list", "hap flags", "rescue maxima"};
     size_t need[kNumScratch];
     scratch_sizes(v, NB, NW, need);
     void* sp[kNumScratch];
@@ -409,8 +411,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
 
     // counters: [0] max hap length, [1] final-list count, [2] error flag,
     // [4 .. 4+NB) per-bin rescue counts, [128 .. 128+kTaskCounters) task
-    // counters of the set launches, [256 .. 256+kHmaxParts) partial maxima
-    static_assert(4 + kMaxBins <= 128 && 128 + kTaskCounters <= 256, "counter layout");
+    // counters of the set launches, [256 .. 256+kHmaxParts) partial maxima,
+    // then kRetrySlots: per-bin retry counts and (last) the exps unit counter
+    static_assert(4 + kMaxBins <= 128 && 128 + kTaskCounters <= 256 && kMaxBins < kRetrySlots,
+                  "counter layout");
     int32_t* sm = static_cast<int32_t*>(c.smallints.p);
     BinParams bp{};
     bp.bd = static_cast<const BatchDesc*>(c.desc.p);
@@ -431,6 +435,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     bp.final_count = sm + 1;
     bp.error_flag = sm + 2;
     bp.rescue_counts = sm + 4;
+    bp.retry_counts = sm + 256 + kHmaxParts;
     bp.task_ctr = sm + 128;
     bp.hapflag = static_cast<uint8_t*>(sp[kHapFlag]);
     bp.items = static_cast<WorkItem*>(sp[kItems]);
@@ -458,6 +463,13 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     fp.rescue_counts = bp.rescue_counts;
     fp.final_items = static_cast<WorkItem*>(sp[kFinalItems]);
     fp.final_count = bp.final_count;
+    // the retry lists take the super-task array, which only the forward
+    // stage reads (SuperTask is at least as large as WorkItem)
+    static_assert(sizeof(SuperTask) >= sizeof(WorkItem), "retry lists in the super-task array");
+    fp.retry_items = static_cast<WorkItem*>(sp[kSuperTasks]);
+    fp.retry_counts = bp.retry_counts;
+    fp.rescue_exps = static_cast<uint8_t*>(sp[kRescueExps]);
+    fp.re_on = range_rescue() ? 1 : 0;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;
@@ -589,12 +601,14 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             bs.bin[1] = NB + b;
             bs.lanes[0] = bs.lanes[1] = tc.lanes;
             tile_set(bs);
-            if (ctr >= kTaskCounters - kF64Sets)
+            if (ctr >= kTaskCounters - kF64Sets - kReSets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
             const int grid = std::min(
                 tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false, roweps()));
-            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fp, bs, grid, st, roweps()),
+            FwdParams fpl = fp;
+            fpl.re_on = fp.re_on && tile_re_supported(tc.rows);
+            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fpl, bs, grid, st, roweps()),
                             "forward kernel")))
                 return rc;
         }
@@ -619,10 +633,12 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                         ++bs.n;
                     }
             tile_set(bs);
-            if (ctr >= kTaskCounters - kF64Sets)
+            if (ctr >= kTaskCounters - kF64Sets - kReSets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
-            if ((rc = check(launch_fwd_tile_set(rows, fast, fp, bs, 0, st, roweps()),
+            FwdParams fpl = fp;
+            fpl.re_on = fp.re_on && tile_re_supported(rows);
+            if ((rc = check(launch_fwd_tile_set(rows, fast, fpl, bs, 0, st, roweps()),
                             "forward kernel")))
                 return rc;
         }
@@ -649,10 +665,14 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
                  "multi-tile kernel");
 }
 
-// Pass 2: FP64 recompute of each tile bin's rescue list with an FP64 tile
-// of at least the bin's size (tiles above 512 rows: one wave per alignment),
-// one launch per FP64 template; pass 3: FP64 waves for anything still not
-// finished.
+// Pass 2 (range_rescue): the rescue lists of tile bins whose rows per lane
+// have a range-extended instance are recomputed in FP32 with per-lane
+// power-of-two scales, one set launch per (rows, form) template; what that
+// pass cannot accept joins the bin's retry list.  Pass 3: FP64 recompute of
+// the retry lists (and of the rescue lists of the other bins) with an FP64
+// tile of at least the bin's size (tiles above 512 rows: one wave per
+// alignment), one launch per FP64 template and list source; pass 4: FP64
+// waves for anything still not finished.
 int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain, hipStream_t st)
 {
     const std::vector<phmm_tile_config>& tiles = *act;
@@ -661,44 +681,95 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     const int64_t n64 = (int64_t)(cur->call_spill_bytes / sizeof(double));
     const int32_t* h_align = cur->h_bin_range + 2 * kMaxWorkBins;
     int rc;
-    // FP64 tile sets by rows per lane and form (a bin's rescue list keeps
-    // the bin's form: FAST-eligible reads recompute in the FAST form).  Each
-    // bin takes the rows per lane nr whose modelled time per alignment is
-    // least: the tile's rows G*nr (G = lanes holding the bin's class) over
-    // the measured slot rate of full nr-row tiles (kF64SlotRate, from the
-    // autotuner), scaled by the lanes a wave leaves idle (64 - (64/G)*G) and
-    // the G fill steps of a 300-column hap.  The 10-row FAST tile (248
-    // VGPRs) still runs two waves per SIMD; the 10-row SAFE one would not,
-    // so SAFE stops at 8.  The small-batch table takes the widest tile of
-    // each form for every bin: a small batch's FP64 time is the tail of each
-    // set launch, so one set per form beats a set per cost choice.
-    // options.f64_rows forces one rows-per-lane choice (the autotuner's
-    // measurement of each).
+    auto re_bin = [&](int b) {
+        const phmm_tile_config& tc = tiles[(size_t)(b / 2)];
+        return fp.re_on != 0 && tile_re_supported(tc.rows);
+    };
+    auto empty_bin = [&](int b) { return plain && h_align[2 * b + 1] <= 0; };
+    // statistics: [0] rescued, [1] of them recomputed in FP64.  The final
+    // list holds, at this point, the one-wave FP32 kernel's results that
+    // need FP64 (the FP64 tile pass adds to it later).
+    int64_t* tot = d_rescued_total;
+    if ((rc = check(launch_add_count(tot, fp.final_count, 1, st), "rescue count")) ||
+        (rc = check(launch_add_count(tot + 1, fp.final_count, 1, st), "rescue count")))
+        return rc;
+    if (fp.re_on) {
+        int nre = 0;
+        for (int b0 = 2 * ntiles - 1; b0 >= 0; --b0) {
+            const int rows = tiles[(size_t)(b0 / 2)].rows;
+            const bool fast = (b0 % 2) == 0;
+            if (!re_bin(b0))
+                continue;
+            // one launch per template: at its largest range-extended bin
+            bool seen = false;
+            for (int b = b0 + 1; b < 2 * ntiles && !seen; ++b)
+                seen = tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast && re_bin(b);
+            if (seen)
+                continue;
+            BinSet bs;
+            for (int b = b0; b >= 0; --b)
+                if (tiles[(size_t)(b / 2)].rows == rows && ((b % 2) == 0) == fast && re_bin(b) &&
+                    !empty_bin(b)) {
+                    bs.bin[bs.n] = b;
+                    bs.lanes[bs.n] = tiles[(size_t)(b / 2)].lanes;
+                    ++bs.n;
+                }
+            if (!bs.n)
+                continue;
+            bs.items = fp.rescue_items;
+            bs.range = bp.align_range;
+            bs.counts = fp.rescue_counts;
+            bs.count_stride = 1;
+            bs.seg = bp.align_range;
+            if (nre >= kReSets)
+                return set_error(PHMM_ERANGE, "too many range-extended set launches");
+            bs.next_task = bp.task_ctr + (kTaskCounters - kF64Sets - kReSets + nre++);
+            if ((rc = check(launch_fwd_tile_set_re(rows, fast, fp, bs, 0, st),
+                            "range-extended kernel")))
+                return rc;
+        }
+    }
+    // FP64 tile sets by rows per lane, form and list (a bin's list keeps the
+    // bin's form: FAST-eligible reads recompute in the FAST form).  Each bin
+    // takes the rows per lane nr whose modelled time per alignment is least:
+    // the tile's rows G*nr (G = lanes holding the bin's class) over the
+    // measured slot rate of full nr-row tiles (kF64SlotRate), scaled by the
+    // lanes a wave leaves idle (64 - (64/G)*G) and the G fill steps of a
+    // 300-column hap.  The 10-row FAST tile (248 VGPRs) still runs two waves
+    // per SIMD; the 10-row SAFE one would not, so SAFE stops at 8.  The
+    // small-batch table takes the widest tile of each form for every bin: a
+    // small batch's FP64 time is the tail of each set launch, so one set per
+    // form beats a set per cost choice.  options.f64_rows forces one
+    // rows-per-lane choice (the autotuner's measurement of each).
     static constexpr int kNr64[] = {4, 6, 8, 10};
     const bool one_set = act == &tiles_small;
     // per-position gcp: the 8-row FP64 tiles in both forms
     const int one_set_fast = roweps() ? kRowEpsRows64 : 10;
-    BinSet set[4][2], wave;  // [rows index][SAFE, FAST]
-    for (auto& row : set)
-        for (BinSet& x : row) {
-            x.items = fp.rescue_items;
-            x.range = bp.align_range;
-            x.counts = fp.rescue_counts;
-            x.count_stride = 1;
-        }
-    wave.items = fp.rescue_items;
-    wave.range = bp.align_range;
-    wave.counts = fp.rescue_counts;
-    wave.count_stride = 1;
+    BinSet set[4][2][2], wave[2];  // [rows index][SAFE, FAST][rescue list, retry list]
+    for (int src = 0; src < 2; ++src) {
+        for (auto& row : set)
+            for (auto& form : row) {
+                BinSet& x = form[src];
+                x.items = src ? fp.retry_items : fp.rescue_items;
+                x.range = bp.align_range;
+                x.counts = src ? fp.retry_counts : fp.rescue_counts;
+                x.count_stride = 1;
+            }
+        wave[src].items = src ? fp.retry_items : fp.rescue_items;
+        wave[src].range = bp.align_range;
+        wave[src].counts = src ? fp.retry_counts : fp.rescue_counts;
+        wave[src].count_stride = 1;
+    }
     for (int b = 2 * ntiles - 1; b >= 0; --b) {
-        if (plain && h_align[2 * b + 1] <= 0)
+        if (empty_bin(b))
             continue;
+        const int src = re_bin(b) ? 1 : 0;
         const int tile = tiles[(size_t)(b / 2)].max_read_len + 1;  // rows the FP64 tile must hold
         const bool fast = b % 2 == 0;
         if (tile > 512) {
-            wave.bin[wave.n] = b;
-            wave.lanes[wave.n] = 64;
-            ++wave.n;
+            wave[src].bin[wave[src].n] = b;
+            wave[src].lanes[wave[src].n] = 64;
+            ++wave[src].n;
             continue;
         }
         int best = -1;
@@ -727,31 +798,36 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
             }
         }
         if (best < 0) {  // no tile holds the class: one wave per alignment
-            wave.bin[wave.n] = b;
-            wave.lanes[wave.n] = 64;
-            ++wave.n;
+            wave[src].bin[wave[src].n] = b;
+            wave[src].lanes[wave[src].n] = 64;
+            ++wave[src].n;
             continue;
         }
-        BinSet& x = set[best][fast];
+        BinSet& x = set[best][fast][src];
         x.bin[x.n] = b;
         x.lanes[x.n] = (tile + kNr64[best] - 1) / kNr64[best];
         ++x.n;
     }
     int nset = 0;
     for (int i = 3; i >= 0; --i)  // more rows per lane first
-        for (int f = 0; f < 2; ++f) {
-            BinSet& x = set[i][f];
-            if (!x.n)
-                continue;
-            x.next_task = bp.task_ctr + (kTaskCounters - kF64Sets + nset++);
-            if ((rc = check(launch_fwd_tile_set_f64(kNr64[i], f == 1, fp, x, 0, st, roweps()),
-                            "FP64 tile kernel")))
-                return rc;
-        }
-    if (wave.n && (rc = check(launch_fwd_wave_f64(fp, wave, bp.hmax, spill64, n64, bp.error_flag,
-                                                  num_cus * 8, st),
-                              "FP64 wave kernel")))
-        return rc;
+        for (int f = 0; f < 2; ++f)
+            for (int src = 0; src < 2; ++src) {
+                BinSet& x = set[i][f][src];
+                if (!x.n)
+                    continue;
+                if (nset >= kF64Sets)
+                    return set_error(PHMM_ERANGE, "too many FP64 set launches");
+                x.next_task = bp.task_ctr + (kTaskCounters - kF64Sets + nset++);
+                if ((rc = check(launch_fwd_tile_set_f64(kNr64[i], f == 1, fp, x, 0, st, roweps()),
+                                "FP64 tile kernel")))
+                    return rc;
+            }
+    for (int src = 0; src < 2; ++src)
+        if (wave[src].n &&
+            (rc = check(launch_fwd_wave_f64(fp, wave[src], bp.hmax, spill64, n64, bp.error_flag,
+                                            num_cus * 8, st),
+                        "FP64 wave kernel")))
+            return rc;
     BinSet fin;
     fin.n = 1;
     fin.bin[0] = 0;
@@ -761,11 +837,20 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     fin.count_stride = 0;
     if ((rc = check(launch_fwd_wave_f64(fp, fin, bp.hmax, spill64, n64, bp.error_flag, num_cus * 8,
                                         st),
-                    "FP64 final kernel")) ||
-        (rc = check(launch_add_count(d_rescued_total, fp.rescue_counts, 2 * ntiles, st),
-                    "rescue count")) ||
-        (rc = check(launch_add_count(d_rescued_total, fp.final_count, 1, st), "rescue count")))
+                    "FP64 final kernel")))
         return rc;
+    // statistics: the rescue lists; in FP64: the retry lists and the rescue
+    // lists of bins without a range-extended pass
+    if ((rc = check(launch_add_count(tot, fp.rescue_counts, 2 * ntiles, st), "rescue count")))
+        return rc;
+    if (!fp.re_on)
+        return check(launch_add_count(tot + 1, fp.rescue_counts, 2 * ntiles, st), "rescue count");
+    if ((rc = check(launch_add_count(tot + 1, fp.retry_counts, 2 * ntiles, st), "rescue count")))
+        return rc;
+    for (int b = 0; b < 2 * ntiles; ++b)  // tables with rows lacking the range-extended pass
+        if (!re_bin(b) &&
+            (rc = check(launch_add_count(tot + 1, fp.rescue_counts + b, 1, st), "rescue count")))
+            return rc;
     return PHMM_OK;
 }
 
@@ -1245,6 +1330,7 @@ extern "C" void phmm_options_default(phmm_options* o)
     o->verbose = 0;
     o->graphs = 1;
     o->small_batch = 1 << 18;
+    o->range_rescue = 1;
 }
 
 extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engine** out)
@@ -1294,8 +1380,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
     if ((rc = e.check(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking), "stream")) ||
         (rc = e.check(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking), "stream")) ||
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
-        (rc = e.check(hipMalloc(&e.d_rescued_total, sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipMemset(e.d_rescued_total, 0, sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipMalloc(&e.d_rescued_total, 2 * sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipMemset(e.d_rescued_total, 0, 2 * sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")) ||
         (rc = e.init_ctx(0)) || (rc = phmm::noalloc_init(device)))
         return bail(rc);
@@ -1325,14 +1411,15 @@ extern "C" int phmm_engine_stats(const phmm_engine* h, phmm_stats* out)
     hipSetDevice(m->e.device);
     hipStreamSynchronize(m->e.stream);
     m->e.finish_timing();
-    int64_t rescued = 0;
+    int64_t rescued[2] = {0, 0};
     if (m->e.d_rescued_total) {
-        if (hipMemcpy(&rescued, m->e.d_rescued_total, sizeof rescued, hipMemcpyDeviceToHost) !=
+        if (hipMemcpy(rescued, m->e.d_rescued_total, sizeof rescued, hipMemcpyDeviceToHost) !=
             hipSuccess)
             hipGetLastError();
     }
     *out = h->e.stats;
-    out->rescued = rescued;
+    out->rescued = rescued[0];
+    out->f64_alignments = rescued[1];
     out->graph_launches = h->e.graph_launches;
     out->graph_captures = h->e.graph_captures;
     return PHMM_OK;
@@ -1350,7 +1437,7 @@ extern "C" int phmm_engine_reset_stats(phmm_engine* h)
     if (h->e.d_rescued_total) {
         hipSetDevice(h->e.device);
         hipDeviceSynchronize();
-        if (hipMemset(h->e.d_rescued_total, 0, sizeof(int64_t)) != hipSuccess)
+        if (hipMemset(h->e.d_rescued_total, 0, 2 * sizeof(int64_t)) != hipSuccess)
             hipGetLastError();
     }
     return PHMM_OK;

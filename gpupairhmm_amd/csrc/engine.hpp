@@ -48,7 +48,7 @@ struct Engine {
     enum ScratchBuf {
         kRows, kHapc, kMeta, kGbCount, kGbStart, kReadPerm, kAlignOff, kKeyCount, kKeyCursor,
         kBinRange, kItems, kAlignRange, kSuperTasks, kRescueItems, kFinalItems, kHapFlag,
-        kNumScratch
+        kRescueExps, kNumScratch
     };
     // An arena must leave at least this much for the spill arena (one block
     // of the FP64 wave kernel holds 32 bytes per column of the longest hap).
@@ -118,7 +118,7 @@ struct Engine {
     };
     std::vector<StageEvents> ev_free, ev_pending;
 
-    int64_t* d_rescued_total = nullptr;  // cumulative FP64 rescues
+    int64_t* d_rescued_total = nullptr;  // cumulative [rescued, of them in FP64]
 
     hipStream_t cap_stream = nullptr;
     int64_t graph_captures = 0, graph_launches = 0;
@@ -160,6 +160,7 @@ struct Engine {
     // Bytes of each batch-sized scratch buffer for view v with NB read bins
     // and NW work bins, and the (NB, NW) of the table a call of v uses.
     static void scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[kNumScratch]);
+    bool range_rescue() const { return opt.range_rescue != 0 && opt.fp64_rescue && !roweps(); }
     void bins_for(const phmm_batch_view& v, int* NB, int* NW) const;
     // Device bytes context k holds (scratch, spill, counters).
     size_t ctx_bytes(int k) const;

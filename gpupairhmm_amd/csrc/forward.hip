@@ -345,7 +345,19 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
 // + n], n counted in rescue_counts[bin]).  T = double is the FP64 recompute
 // of those lists (start constant 2^1000/H); a result whose sum is not far
 // above double's floor goes to the final list.
-template <typename T, int NR, bool FAST, bool ROWEPS>
+//
+// RE (T = float): the range-extended recompute of the rescue lists.  Lane t
+// scales its values by 2^E_t (FwdParams: kReShift and the rule for E_t) --
+// the mask multiply that injects the boundary on a group's first lane
+// carries the factor 2^(E_t - E_{t-1}) on the others, so the sweep itself is
+// unchanged -- and an accepted sum a is exactly a * 2^(kReShift - E_R) of
+// the FP32 pass's units.  The magnitude of a lane's last row is read from
+// its accumulator: every lane adds up the chain it passes down (only the
+// result lane's sum is a score), so a lane above the result lane holds the
+// sum over the columns of its last row's outgoing chain -- at no cost in
+// the sweep.  The first pass keeps its ilogb for the recompute's scales,
+// the recompute checks them with it.
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false>
 __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
                                            const TileRead<T, NR, ROWEPS>& R, bool active, int G,
                                            int t, int HL, int hoff, const WorkItem& it,
@@ -361,6 +373,15 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     const T beta = R.beta;
     const int L = R.L;
     const float nl = R.nl;
+    constexpr bool REC = F32 && !ROWEPS && re_rows(NR);  // record per-lane magnitudes
+    static_assert(!RE || REC, "the range-extended pass is an FP32 pass of the one-eps tiles");
+    // the group's lane that forms the sum, the group's first lane and its
+    // lanes as a mask (computed where used: nothing extra stays live in
+    // the sweep)
+    auto res_lane = [&]() { return L / NR; };
+    auto group_mask = [&]() {
+        return G >= 64 ? ~0ull : (((1ull << G) - 1ull) << (lane - t));
+    };
 
     // ---- wavefront sweep -------------------------------------------------
     T M[NR], D[NR];
@@ -380,7 +401,48 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     // boundary injection: a group's first lane takes row 0 (M = I = 0,
     // D = C0) instead of its left neighbour's last row -- as a multiply by a
     // 0/1 lane mask, which issues at the full FP32 rate where a select does not
-    const T nf = first ? (T)0 : (T)1;
+    T nf = first ? (T)0 : (T)1;
+    int re_E = 0, re_dE = 0;
+    bool re_bad = false;
+    if constexpr (RE) {
+        // per-lane exponents from the FP32 pass's maxima r_k (all lanes of
+        // the wave take part in the ballot and shuffles)
+        const int slot = active ? it.pad : -1;  // tile_items_task: the rescue-list slot
+        const int cm = active && HL > 0 ? ilogbf((float)C0) : 0;
+        const int res_t = res_lane(), gbase = lane - t;
+        const unsigned long long gmask = group_mask();
+        int r = 127;
+        if (active && slot >= 0 && t < res_t)
+            r = t < kReKnownLanes ? (int)(int8_t)p.rescue_exps[(int64_t)slot * kReKnownLanes + t]
+                                  : -128;  // not recorded: extrapolated like an underflow
+        re_bad = active && slot < 0;
+        const unsigned long long inv = __ballot(active && t < res_t && r < kReValid) & gmask;
+        const int lv = (inv ? __ffsll((long long)inv) - 1 - gbase : res_t) - 1;  // last known lane
+        const int r_lv = __shfl(r, gbase + (lv > 0 ? lv : 0), 64);
+        const int r_lv1 = __shfl(r, gbase + (lv > 1 ? lv - 1 : 0), 64);
+        // lanes past the result lane (rows past the read) keep its scale
+        const int te = t < res_t ? t : res_t;
+        const int r_prev = __shfl(r, gbase + (te > 0 ? te - 1 : 0), 64);
+        int E = 0;
+        if (te >= 1) {
+            if (te - 1 <= lv) {
+                E = cm - r_prev;
+            } else {
+                int slope = lv >= 1 ? r_lv - r_lv1 : lv == 0 ? r_lv - cm : kReUnknownSlope;
+                slope = slope < 0 ? slope : 0;
+                const int e1 = lv >= 0 ? cm - r_lv : 0;  // E of lane lv + 1
+                E = e1 - slope * (te - lv - 1);
+            }
+        }
+        const int Ep = __shfl(E, lane > 0 ? lane - 1 : 0, 64);
+        re_dE = t >= 1 ? E - Ep : 0;
+        if (active && t <= res_t && (re_dE > 126 || re_dE < -126))
+            re_bad = true;
+        re_E = E;
+        if (!first)
+            nf = ldexpf(1.0f, re_dE < -126 ? -126 : re_dE > 126 ? 126 : re_dE);
+        C0 = ldexpf((float)C0, -kReShift);
+    }
     const T c0f = first ? C0 : (T)0;
     // M and D of the row above reach a lane two steps after they were
     // computed (diagonal dependency), I one step after: step u reads ring
@@ -482,9 +544,23 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
                 acc += (s4 + u <= acc_last) ? chain : (T)0;
             else
                 acc += chain;
-            Mr[cur] = TT::shr(M[NR - 1]) * nf;
-            Dr[cur] = fma(TT::shr(D[NR - 1]), nf, c0f);
-            Iin = TT::shr(chain) * nf;
+            if constexpr (RE) {
+                // a group's first lane must not see its neighbour's values:
+                // scaled by that alignment's lane factors they may overflow,
+                // and inf * 0 would poison this group.  The shifted value is
+                // clamped to FLT_MAX first (v_min also maps NaN to it), then
+                // masked as in the first pass.  (Not a select: the compiler
+                // turns a select of a DPP result into an exec-masked DPP,
+                // which reads 0 from the masked-off source lanes.)
+                constexpr float kBig = 3.4028234663852886e38f;
+                Mr[cur] = fminf(TT::shr(M[NR - 1]), kBig) * nf;
+                Dr[cur] = fma(fminf(TT::shr(D[NR - 1]), kBig), nf, c0f);
+                Iin = fminf(TT::shr(chain), kBig) * nf;
+            } else {
+                Mr[cur] = TT::shr(M[NR - 1]) * nf;
+                Dr[cur] = fma(TT::shr(D[NR - 1]), nf, c0f);
+                Iin = TT::shr(chain) * nf;
+            }
         }
         letters = next;
         wlo = whi;
@@ -497,6 +573,42 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     for (; s4 < nsteps; s4 += 4)
         sweep4(s4, true);
 
+    if constexpr (RE) {
+        // each lane's incoming magnitude, in its own scale: the lane above's
+        // chain sum times 2^(E_t - E_{t-1}), within the window
+        const float rmp = shr1f((float)acc);
+        const int res_t = res_lane();
+        bool bad = re_bad;
+        if (active && t >= 1 && t <= res_t) {
+            if (!(rmp > 0.0f && rmp <= 3.4028234663852886e38f)) {
+                bad = true;
+            } else {
+                const int li = ilogbf(rmp) + re_dE;
+                const int cre = ilogbf((float)C0);
+                bad = bad || li < cre - kReLow || li > cre + kReHigh;
+            }
+        }
+        const unsigned long long fb = __ballot(bad) & group_mask();
+        if (active && t == res_t) {
+            const float a = (float)acc;
+            if (!fb && a >= 0x1p-100f && a <= 3.4028234663852886e38f) {
+                // a * 2^(kReShift - E) = m * 2^K exactly: the score does not
+                // depend on how the rows were split into lanes
+                int e2 = 0;
+                const float m = frexpf(a, &e2);
+                const int K = e2 + kReShift - re_E;
+                const double off = FAST ? (double)kLog10Scale - (double)kLog10FastKappa
+                                        : (double)kLog10Scale + log10((double)(float)beta);
+                bd.scores[it.out] =
+                    (float)(log10((double)m) + (double)K * 0.30102999566398119521 - off);
+            } else {
+                const int n = atomicAdd(p.retry_counts + bin, 1);
+                p.retry_items[rescue_base + n] = it;
+            }
+        }
+        return;
+    }
+    int slot = -1;  // REC: the rescued alignment's rescue-list slot
     if (active && t == L / NR) {
         if (F32) {
             const float a = (float)acc;
@@ -512,6 +624,8 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
             if (p.rescue_on && !(raw >= p.rescue_min)) {
                 const int n = atomicAdd(p.rescue_counts + bin, 1);
                 p.rescue_items[rescue_base + n] = it;
+                if (REC && p.re_on)
+                    slot = rescue_base + n;
             }
         } else {
             const double a = (double)acc;
@@ -527,11 +641,22 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
             }
         }
     }
+    if constexpr (REC) {
+        // the group's lanes above the result lane store their magnitudes for
+        // the range-extended pass
+        const int res_t = res_lane();
+        const int gs = __shfl(slot, lane - t + (res_t < G ? res_t : G - 1), 64);
+        if (gs >= 0 && active && t < res_t && t < kReKnownLanes) {
+            const int e = acc > 0.0f ? ilogbf((float)acc) : -128;
+            p.rescue_exps[(int64_t)gs * kReKnownLanes + t] =
+                (uint8_t)(int8_t)(e < -128 ? -128 : e > 127 ? 127 : e);
+        }
+    }
 }
 
 // A task of 64/G consecutive alignments of an item list (the FP64 recompute
 // of rescue lists): every group sets up its own read and sweeps its own hap.
-template <typename T, int NR, bool FAST, bool ROWEPS>
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false>
 __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const WorkItem* items, int begin, int count, int G,
                                                 int task, int rescue_base, int bin)
@@ -548,13 +673,16 @@ __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename Til
     it.hap = 0;
     it.out = -1;
     it.pad = 0;
-    if (active)
+    if (active) {
         it = items[begin + local];
+        if (RE)
+            it.pad = begin + local;  // the rescue-list slot: its maxima in rescue_exps
+    }
     TileRead<T, NR, ROWEPS> R;
     tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, it.read, t, R);
     const int HL = active ? bd.hap_len[it.hap] : 0;
     const int hoff = active ? bd.hap_off[it.hap] : 0;
-    tile_sweep<T, NR, FAST, ROWEPS>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
+    tile_sweep<T, NR, FAST, ROWEPS, RE>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
 }
 
 // A super-task: the groups set up their reads once and sweep the segment's
@@ -649,15 +777,17 @@ __device__ __forceinline__ int next_task(int32_t* ctr)
 // 13-row FAST tile fit three (at most 168 VGPRs, no spills) and issue more
 // of their dependent FMA chains per cycle that way; the larger tiles need
 // up to 256 VGPRs and run two.
-template <typename T, int NR, bool FAST>
+// The range-extended instances (a smaller share of the time, and a few more
+// live values in setup) are not held to three.
+template <typename T, int NR, bool FAST, bool RE>
 constexpr int tile_waves_per_simd()
 {
-    return (sizeof(T) == 4 && (NR == 12 || (NR == 13 && FAST))) ? 3 : 1;
+    return (sizeof(T) == 4 && !RE && (NR == 12 || (NR == 13 && FAST))) ? 3 : 1;
 }
 
-template <typename T, int NR, bool FAST, bool ROWEPS = false>
+template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST>(), 8))) void k_fwd_tile_set(
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE>(), 8))) void k_fwd_tile_set(
     FwdParams p, BinSet bs)
 {
     static_assert(NR >= 4 && NR <= 32, "rows per lane");
@@ -667,20 +797,25 @@ __attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST>(), 8))) void
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
         return;
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
+        // the task's bin, in scalar registers (uniform across the wave)
         const int k = set_find(sl, task);
-        const int bin = __shfl(sl.bin, k, 64);
-        const int G = __shfl(sl.G, k, 64);
-        const int begin = __shfl(sl.begin, k, 64);
-        const int count = __shfl(sl.count, k, 64);
-        const int t0 = __shfl(sl.incl - sl.ntask, k, 64);
+        const int bin = __builtin_amdgcn_readfirstlane(__shfl(sl.bin, k, 64));
+        const int G = __builtin_amdgcn_readfirstlane(__shfl(sl.G, k, 64));
+        const int begin = __builtin_amdgcn_readfirstlane(__shfl(sl.begin, k, 64));
+        const int count = __builtin_amdgcn_readfirstlane(__shfl(sl.count, k, 64));
+        const int t0 = __builtin_amdgcn_readfirstlane(__shfl(sl.incl - sl.ntask, k, 64));
         // an item bin's results below the threshold join its tile bin's list
         const int rbin = bin >= bs.item_bin0 ? bin - bs.item_bin0 : bin;
-        const int rescue_base = bs.seg ? bs.seg[2 * rbin] : 0;
-        if (bs.st && bin < bs.item_bin0)
-            tile_super_task<T, NR, FAST, ROWEPS>(p, tab, bs.st[begin + (task - t0)], G,
-                                                 rescue_base, rbin);
-        else
-            tile_items_task<T, NR, FAST, ROWEPS>(p, tab, bs.items, begin, count, G, task - t0,
+        const int rescue_base =
+            __builtin_amdgcn_readfirstlane(bs.seg ? bs.seg[2 * rbin] : 0);
+        if constexpr (!RE) {
+            if (bs.st && bin < bs.item_bin0) {
+                tile_super_task<T, NR, FAST, ROWEPS>(p, tab, bs.st[begin + (task - t0)], G,
+                                                     rescue_base, rbin);
+                continue;
+            }
+        }
+        tile_items_task<T, NR, FAST, ROWEPS, RE>(p, tab, bs.items, begin, count, G, task - t0,
                                                  rescue_base, rbin);
     }
 }
@@ -969,10 +1104,10 @@ static int resident_blocks_per_cu(K kernel)
     return n > 0 ? n : 4;
 }
 
-template <typename T, int NR, bool FAST, bool ROWEPS = false>
+template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false>
 static int set_blocks_per_cu()
 {
-    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST, ROWEPS>);
+    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST, ROWEPS, RE>);
     return n;
 }
 
@@ -994,25 +1129,42 @@ static int grid_of(int grid, int per_cu)
     return cus * per_cu;
 }
 
-template <typename T, int NR, bool ROWEPS = false>
+template <typename T, int NR, bool ROWEPS = false, bool RE = false>
 static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                 hipStream_t st)
 {
     if (bs.n < 1 || bs.n > 64 || !bs.counts || !bs.next_task || (!bs.items && !bs.st))
         return hipErrorInvalidValue;
+    if (RE && (bs.st || !bs.seg || !p.retry_items || !p.retry_counts || !p.rescue_exps))
+        return hipErrorInvalidValue;
     for (int i = 0; i < bs.n; ++i)
         if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
             return hipErrorInvalidValue;
     if (fast) {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true, ROWEPS>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true, ROWEPS>())), dim3(64),
-                           0, st, p, bs);
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true, ROWEPS, RE>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true, ROWEPS, RE>())),
+                           dim3(64), 0, st, p, bs);
     } else {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false, ROWEPS>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false, ROWEPS>())), dim3(64),
-                           0, st, p, bs);
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false, ROWEPS, RE>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false, ROWEPS, RE>())),
+                           dim3(64), 0, st, p, bs);
     }
     return hipGetLastError();
+}
+
+bool tile_re_supported(int NR) { return re_rows(NR); }
+
+hipError_t launch_fwd_tile_set_re(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                  int grid, hipStream_t st)
+{
+    switch (NR) {
+    case 12: return launch_set_nr<float, 12, false, true>(fast, p, bs, grid, st);
+    case 13: return launch_set_nr<float, 13, false, true>(fast, p, bs, grid, st);
+    case 16: return launch_set_nr<float, 16, false, true>(fast, p, bs, grid, st);
+    case 19: return launch_set_nr<float, 19, false, true>(fast, p, bs, grid, st);
+    case 20: return launch_set_nr<float, 20, false, true>(fast, p, bs, grid, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinSet& bs, int grid,

@@ -62,8 +62,10 @@ __host__ __device__ inline bool tile_rows_supported(int r)
 {
     return (r >= 4 && r <= 32 && r % 4 == 0) || r == 13 || r == 19;
 }
-constexpr int kTaskCounters = 96;   // set launches per call (plain mode: one per bin)
-constexpr int kF64Sets = 8;         // of them, the last ones: FP64 tile sets
+constexpr int kTaskCounters = 128;  // set launches per call (plain mode: one per bin)
+constexpr int kF64Sets = 16;        // of them, the last ones: FP64 tile sets
+constexpr int kReSets = 24;         // before those: range-extended FP32 sets
+constexpr int kRetrySlots = 128;    // retry counts per bin
 constexpr int kHapBuckets = 3072;   // hap-length buckets of the sort key
 constexpr int kKeyParts = 8;        // counters per (bin, bucket): groups spread over them
 constexpr int kKeysPerBin = kHapBuckets * kKeyParts;
@@ -152,11 +154,51 @@ struct FwdParams {
     int32_t* rescue_counts;      // per bin
     WorkItem* final_items;       // alignments the FP64 tile pass could not finish
     int32_t* final_count;
+    // Range-extended FP32 pass (re_on): the FP32 pass records, for the first
+    // kReKnownLanes lanes of a rescued alignment, ilogb of each lane's last
+    // row's magnitude (rescue_exps: 16 bytes per rescue-list slot, the
+    // slot's index as in rescue_items; lanes past them count as unknown); the pass recomputes the rescue
+    // lists with per-lane power-of-two scales and sends what it cannot
+    // accept to the retry lists (per tile bin, like the rescue lists), which
+    // the FP64 tile pass recomputes.
+    WorkItem* retry_items;
+    int32_t* retry_counts;
+    uint8_t* rescue_exps;
+    int32_t re_on;
+    int32_t pad0;
     float rescue_min;            // FP32 sum (reference units) threshold
     int32_t rescue_on;
     int32_t semantics;           // 0: reference GPU, 1: reference CPU, 2: per-position gcp
     int32_t pad;
 };
+
+// Range-extended FP32 pass.  Lane t of a group scales its values by 2^E_t
+// (E_0 = 0) and multiplies what it takes from lane t-1 by 2^(E_t - E_{t-1});
+// the start constant is the FP32 pass's times 2^-kReShift.  E_t is chosen
+// from the FP32 pass's per-lane magnitudes r_k (ilogb): E_t = ilogb(start) -
+// r_{t-1} while r_0 .. r_{t-1} are all known (>= kReValid), beyond that the
+// last known per-lane decay continues.  A result is accepted when every
+// lane's incoming magnitude lies within [start/2^kReLow, start*2^kReHigh]
+// (in its own scale), the lane-to-lane factors fit a float and the sum is
+// finite and at least 2^kReMinSum.  The window trades two risks: a lane
+// scaled too high overflows (detected: the sum or a magnitude is not
+// finite), a lane scaled too low flushes values that a path with a better
+// suffix would have needed (not detectable locally).  kReShift leaves 30
+// bits above the first pass's own headroom for extrapolation errors;
+// kReLow keeps every lane's values within 2^70 of the first pass's start,
+// i.e. at least about 2^150 above the FP32 floor.
+constexpr int kReShift = 30;
+constexpr int kReValid = -120;
+constexpr int kReLow = 40;
+constexpr int kReHigh = 50;
+constexpr int kReMinSum = -100;
+constexpr int kReUnknownSlope = -150;  // per-lane decay when lane 0's maximum is unknown
+constexpr int kReKnownLanes = 16;      // lanes per alignment with a recorded magnitude
+// Rows per lane with a range-extended instance (the tuned tables' rows).
+__host__ __device__ constexpr bool re_rows(int NR)
+{
+    return NR == 12 || NR == 13 || NR == 16 || NR == 19 || NR == 20;
+}
 
 // Everything the binning kernels read and write (binning.hip).
 struct BinParams {
@@ -177,6 +219,7 @@ struct BinParams {
     int32_t* hmax;             // [1]: the longest hap (k_scan_bins folds hmax_part)
     int32_t* hmax_part;        // [kHmaxParts]: per-block maxima of the hap pass
     int32_t* rescue_counts;    // [NB]
+    int32_t* retry_counts;     // [kRetrySlots]: retry lists' counts per bin
     int32_t* final_count;      // [1]
     int32_t* error_flag;       // [1]
     int32_t* task_ctr;         // [kTaskCounters]: task counters of the set launches
@@ -234,6 +277,12 @@ hipError_t launch_fwd_tile_set(int NR, bool fast, const FwdParams& p, const BinS
                                hipStream_t st, bool roweps = false);
 hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const BinSet& bs,
                                    int grid, hipStream_t st, bool roweps = false);
+// The range-extended FP32 pass over rescue lists (item bins of a BinSet
+// whose seg gives each bin's retry-list base); rows per lane with such an
+// instance: tile_re_supported.
+bool tile_re_supported(int NR);
+hipError_t launch_fwd_tile_set_re(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                  int grid, hipStream_t st);
 hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
                                float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);

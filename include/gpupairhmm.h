@@ -87,18 +87,29 @@ typedef struct phmm_options {
     int32_t graphs;            /* PHMM_MODE_BATCHED: 1 (default) replays the
                                   call's kernel sequence as one captured HIP
                                   graph; 0 launches the kernels one by one */
-    int32_t small_batch;       /* PHMM_MODE_BATCHED: batches of at most this
-                                  many alignments use a one-template tile
-                                  table (16 rows per lane for every class),
-                                  so their latency is one launch per stage
-                                  rather than one per tile template; results
-                                  are bit-identical.  0: always the tuned
-                                  table.  Default 1 << 18. */
+    int32_t small_batch;       /* batches of at most this many alignments
+                                  (either mode) use a one-template tile table
+                                  (16 rows per lane for every class), so their
+                                  latency is one launch per stage rather than
+                                  one per tile template.  Scores are
+                                  bit-identical across tables except where the
+                                  range-extended pass accepts an alignment on
+                                  one table and hands it to FP64 on the other
+                                  (both within 5e-5 log10 of the FP64
+                                  oracle).  0: always the tuned table.
+                                  Default 1 << 18. */
     int32_t f64_rows;          /* rows per lane of every FP64 rescue tile
                                   (4, 6, 8 or 10; SAFE tiles at most 8); 0
                                   (default): chosen per bin from the
                                   measured FP64 tile rates */
-    int32_t reserved[3];
+    int32_t range_rescue;      /* 1 (default): results below rescue_min_sum
+                                  are first recomputed in FP32 with a power-
+                                  of-two scale per tile lane, chosen from the
+                                  first pass's per-lane maxima; only what
+                                  that pass cannot accept goes to FP64.
+                                  0: every such result is recomputed in FP64.
+                                  Not used under PHMM_SEM_GCP. */
+    int32_t reserved[2];
 } phmm_options;
 
 /* ---- batch view (BatchOfSequenceGroups, common.cuh:109-201) --------- *
@@ -143,7 +154,8 @@ typedef struct phmm_server phmm_server;
 typedef struct phmm_stats {
     int64_t alignments;        /* alignments scored */
     int64_t cells;             /* sum of readLen*hapLen (gpuworker.cuh:495-520) */
-    int64_t rescued;           /* alignments recomputed in FP64 */
+    int64_t rescued;           /* alignments recomputed after the FP32 pass
+                                  (range-extended FP32 or FP64) */
     int64_t fast_alignments;   /* alignments on the rescaled 5-op kernel */
     int64_t multitile_alignments;
     double engine_seconds;     /* wall time inside phmm_score_batch */
@@ -151,8 +163,10 @@ typedef struct phmm_stats {
     int64_t graph_launches;    /* HIP graph replays (batched mode) */
     int64_t graph_captures;    /* graphs captured (first call, new scratch
                                   sizes or a new tile table) */
-    double rescue_seconds;     /* event-timed FP64 rescue stage (if timed) */
-    int64_t reserved[5];
+    double rescue_seconds;     /* event-timed rescue stage (if timed) */
+    int64_t f64_alignments;    /* of the rescued alignments, those recomputed
+                                  in FP64 */
+    int64_t reserved[4];
 } phmm_stats;
 
 /* One row of the tile table (the reference's per-architecture

@@ -31,7 +31,7 @@ def _bits(x):
 def test_graph_replay_matches_eager_and_plain(lib, timed, small):
     import gpupairhmm_amd as G
     bs = _batches()
-    with G.Engine(mode=G.MODE_PLAIN) as plain, \
+    with G.Engine(mode=G.MODE_PLAIN, small_batch=small) as plain, \
             G.Engine(mode=G.MODE_BATCHED, graphs=False, small_batch=small) as eager, \
             G.Engine(mode=G.MODE_BATCHED, graphs=True, time_kernels=timed,
                      small_batch=small) as graph:

@@ -11,6 +11,7 @@ import pytest
 from conftest import say
 
 TOL = 1e-3  # absolute, log10
+FLOOR = -75.26  # log10(FLT_MIN) - log10(FLT_MAX/16): scores below it are rescued
 
 pytestmark = pytest.mark.gpu
 
@@ -122,7 +123,7 @@ def test_every_tile_class_and_form_in_one_call(lib, oracle, small_batch):
             f"{b.num_alignments} alignments")
         got = e.score_batch(b)
         st = e.stats()
-    with G.Engine(mode=G.MODE_PLAIN) as p:
+    with G.Engine(mode=G.MODE_PLAIN, small_batch=small_batch) as p:
         plain = p.score_batch(b)
         pst = p.stats()
     ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
@@ -144,16 +145,26 @@ def test_plain_and_batched_identical(engine, plain_engine):
     assert np.array_equal(a.view(np.uint32), p.view(np.uint32))
 
 
-def test_small_batch_table_identical(lib, engine):
-    """Small batched calls run on the one-template table (16 rows per lane);
-    the tuned table must give the same bits."""
+def _same_but_rescued(a, b):
+    """Scores of one batch on two tile tables: bit-identical above the FP32
+    rescue floor; below it the range-extended pass may accept an alignment on
+    one table and hand it to FP64 on the other (both FP32-accurate)."""
+    hi = (a > FLOOR + 0.5) & (b > FLOOR + 0.5)
+    return (np.array_equal(a[hi].view(np.uint32), b[hi].view(np.uint32)) and
+            _max_delta(a[~hi], b[~hi]) <= 5e-5)
+
+
+def test_small_batch_table_agrees(lib, engine):
+    """Small calls run on the one-template table (16 rows per lane); the
+    tuned table gives the same bits above the rescue floor."""
     import gpupairhmm_amd as G
     b = G.HostBatch.synthetic(2, seed=78, num_records=6)
     say(f"case: small-batch table vs tuned table, {b.num_alignments} alignments")
     a = engine.score_batch(b)
     with G.Engine(small_batch=0) as e2:
         t = e2.score_batch(b)
-    assert np.array_equal(a.view(np.uint32), t.view(np.uint32))
+    assert _same_but_rescued(a, t)
+    print(f"identical: {np.mean(a.view(np.uint32) == t.view(np.uint32)):.4f}")
 
 
 @pytest.mark.parametrize("tb", [1, 7, 1000, 0])
@@ -233,7 +244,7 @@ def test_record_with_many_haps(engine, plain_engine, oracle):
     with G.Engine(small_batch=0) as e2:
         tuned = e2.score_batch(b)
     plain = plain_engine.score_batch(b)
-    assert np.array_equal(got.view(np.uint32), tuned.view(np.uint32))
+    assert _same_but_rescued(got, tuned)
     assert np.array_equal(got.view(np.uint32), plain.view(np.uint32))
 
 

@@ -46,7 +46,7 @@ def test_tile_shapes(lib, oracle, rows):
         b = _batch(G, max_len, seed=rows * 100 + lanes)
         say(f"case: tile {lanes}x{rows}, reads up to {max_len}, {b.num_alignments} alignments")
         ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
-        with G.Engine(mode=G.MODE_PLAIN) as e:
+        with G.Engine(mode=G.MODE_PLAIN, small_batch=0) as e:  # this table, at any size
             e.set_tiles([(lanes, rows, max_len)])
             got = e.score_batch(b)
             st = e.stats()
