@@ -410,21 +410,28 @@ def main():
     value = cells * args.steps / dt / 1e9
     kernel_avg = st.kernel_seconds / args.steps if st.kernel_seconds > 0 else None
     achieved = cells * FLOP_PER_CELL / kernel_avg / 1e12 if kernel_avg else None
-    traffic = None
+    traffic = traffic_step = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_step")
+            prof = json.load(f)
+        if "dominant_kernel_fetch_bytes_per_launch" in prof:
+            traffic = (prof["dominant_kernel_fetch_bytes_per_launch"] +
+                       prof["dominant_kernel_write_bytes_per_launch"])
+        traffic_step = prof.get("hbm_bytes_per_step")
     roofline = {"bound": "valu", "achieved": round(achieved, 2) if achieved else None,
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4) if achieved else None,
                 "traffic": traffic,
+                "traffic_kernel": "k_fwd_tile_set<float, 19, true> (one launch per step)",
+                "traffic_per_step": traffic_step,
                 "kernel": "k_fwd_tile_set<float> (the FP32 forward stage of one step: one "
                           "persistent launch per tile template, event-timed together)",
                 "kernel_ms_per_step": round(kernel_avg * 1e3, 3) if kernel_avg else None,
                 "flop_per_cell": FLOP_PER_CELL,
                 "note": "compute-bound on FP32 VALU (a 3-term recurrence, no MFMA); peak is "
-                        "MI355X's FP32 157.3 TFLOP/s; traffic is HBM bytes per step from "
-                        "rocprofv3 FETCH_SIZE + WRITE_SIZE"}
+                        "MI355X's FP32 157.3 TFLOP/s; traffic is HBM bytes per launch of the "
+                        "dominant kernel (and per step, all kernels) from rocprofv3 FETCH_SIZE + "
+                        "WRITE_SIZE, calibrated with a copy kernel"}
     result = {
         "metric": METRIC,
         "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,

@@ -67,6 +67,8 @@ Engine::~Engine()
     hipSetDevice(device);
     if (stream)
         hipStreamSynchronize(stream);
+    if (stream2)
+        hipStreamSynchronize(stream2);
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
     for (auto& s : slots)
@@ -89,6 +91,20 @@ Engine::~Engine()
         hipStreamDestroy(stream);
     if (copy_stream)
         hipStreamDestroy(copy_stream);
+    if (stream2)
+        hipStreamDestroy(stream2);
+}
+
+int Engine::create_queue_stream(hipStream_t* out)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
+        std::vector<uint32_t> mask((size_t)(prop.multiProcessorCount + 31) / 32, 0xffffffffu);
+        if (hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data()) == hipSuccess)
+            return PHMM_OK;
+    }
+    (void)hipGetLastError();
+    return check(hipStreamCreateWithFlags(out, hipStreamNonBlocking), "stream");
 }
 
 void Engine::free_ctx(CallCtx& c)
@@ -315,9 +331,8 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     // keeps the per-class launches of the tuned table); per-position gcp
     // always does: its tile kernels (a gap continuation register per row)
     // exist for 16 rows per lane
-    act = (roweps() || (opt.small_batch > 0 && v.num_alignments <= opt.small_batch))
-              ? &tiles_small
-              : &tiles;
+    const int64_t table_n = table_hint >= 0 ? table_hint : v.num_alignments;
+    act = (roweps() || (opt.small_batch > 0 && table_n <= opt.small_batch)) ? &tiles_small : &tiles;
     const std::vector<phmm_tile_config>& tiles = *act;
     const int ntiles = (int)tiles.size();
     const int NB = 2 * ntiles + 2;  // + multi-tile bin + exact bin
@@ -487,7 +502,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
         }
     }
     const bool plain = opt.mode == PHMM_MODE_PLAIN;
-    if (plain || !opt.graphs || arena) {
+    if (plain || !opt.graphs || arena || eager_hint) {
         if ((rc = enqueue_prep(bp, hd, false, st)))
             return rc;
         if (plain) {
@@ -936,17 +951,12 @@ void Engine::free_slot(Slot& s)
     s = Slot{};
 }
 
-int Engine::launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1)
-{
-    const ChunkPart part{&v, g0, g1};
-    return launch_parts(s, &part, 1);
-}
-
 // Stages groups of one or more host views, in order, into slot s as one
-// self-contained batch (offsets and prefixes rebased) and enqueues copy-in,
-// scoring and copy-out; s.compute_done fires when s.pin_scores holds the
-// scores.
-int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
+// self-contained batch (offsets and prefixes rebased) and enqueues the
+// copy-in; s.h2d_done fires when it landed.  compute_slot then enqueues the
+// scoring (on the slot's context and stream, behind the copy) and the
+// copy-out; s.compute_done fires when s.pin_scores holds the scores.
+int Engine::stage_parts(Slot& s, const ChunkPart* parts, int nparts, int64_t table_n)
 {
     struct Span {
         int r0, r1, h0, h1;
@@ -1151,12 +1161,11 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
                                        hipMemcpyHostToDevice, copy_stream),
                         "H2D copy")))
             break;
-    if (rc ||
-        (rc = check(hipEventRecord(s.h2d_done, copy_stream), "event")) ||
-        (rc = check(hipStreamWaitEvent(stream, s.h2d_done, 0), "event wait")))
+    if (rc || (rc = check(hipEventRecord(s.h2d_done, copy_stream), "event")))
         return rc;
     char* D = static_cast<char*>(s.dev.p);
-    phmm_batch_view dv{};
+    phmm_batch_view& dv = s.dv;
+    dv = phmm_batch_view{};
     dv.num_groups = (int32_t)G;
     dv.num_reads = (int32_t)nR;
     dv.num_haps = (int32_t)nH;
@@ -1177,15 +1186,30 @@ int Engine::launch_parts(Slot& s, const ChunkPart* parts, int nparts)
     dv.num_haps_per_group = reinterpret_cast<const int32_t*>(D + L.nh);
     dv.reads_prefix = reinterpret_cast<const int32_t*>(D + L.rp);
     dv.haps_prefix = reinterpret_cast<const int32_t*>(D + L.hp);
-    if ((rc = score_device(dv, static_cast<float*>(s.scores.p), stream)))
-        return rc;
-    if ((rc = check(hipMemcpyAsync(s.pin_scores.p, s.scores.p, sizeof(float) * (size_t)nA,
-                                   hipMemcpyDeviceToHost, stream),
-                    "D2H copy")) ||
-        (rc = check(hipEventRecord(s.compute_done, stream), "event")))
-        return rc;
-    s.busy = true;
+    s.table_n = table_n;
     s.n_scores = nA;
+    s.busy = true;
+    return PHMM_OK;
+}
+
+int Engine::compute_slot(Slot& s)
+{
+    hipStream_t st = s.st ? s.st : stream;
+    int rc;
+    if ((rc = check(hipStreamWaitEvent(st, s.h2d_done, 0), "event wait")))
+        return rc;
+    table_hint = s.table_n;
+    eager_hint = s.eager;
+    rc = score_device(s.dv, static_cast<float*>(s.scores.p), st, nullptr, 0, s.ctx);
+    table_hint = -1;
+    eager_hint = false;
+    if (rc)
+        return rc;
+    if ((rc = check(hipMemcpyAsync(s.pin_scores.p, s.scores.p, sizeof(float) * (size_t)s.n_scores,
+                                   hipMemcpyDeviceToHost, st),
+                    "D2H copy")) ||
+        (rc = check(hipEventRecord(s.compute_done, st), "event")))
+        return rc;
     return PHMM_OK;
 }
 
@@ -1222,36 +1246,59 @@ int Engine::score_batch(const phmm_batch_view& v, float* out)
     }
     // Pieces of a transfer chunk: a large chunk is staged, copied and scored
     // in up to four record-aligned pieces, so the copies of piece k+1 overlap
-    // the kernels of piece k (scores do not depend on the cut).
-    {
-        std::vector<std::pair<int, int>> pieces;
-        for (const auto& ch : chunks) {
-            int64_t na = 0;
-            for (int g = ch.first; g < ch.second; ++g)
-                na += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
-            // pieces of at least 3e5 alignments keep the tuned tile table
-            // (calls of up to options.small_batch use the one-template one)
-            const int n = (int)std::max<int64_t>(1, std::min<int64_t>(4, na / 300000));
-            int g0 = ch.first;
-            int64_t acc = 0;
-            int k = 1;
-            for (int g = ch.first; g < ch.second && k < n; ++g) {
-                acc += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
-                if (acc * n >= na * k) {
-                    pieces.push_back({g0, g + 1});
-                    g0 = g + 1;
-                    ++k;
-                }
+    // the kernels of piece k.  The first piece is half the size of the
+    // others (its copy is the one nothing hides).  Every piece scores on the
+    // tile table of its whole chunk, so the scores do not depend on the cut.
+    struct Piece {
+        int g0, g1;
+        int64_t table_n;
+    };
+    std::vector<Piece> pieces;
+    for (const auto& ch : chunks) {
+        int64_t na = 0;
+        for (int g = ch.first; g < ch.second; ++g)
+            na += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+        const int n = (int)std::max<int64_t>(1, std::min<int64_t>(4, na / 250000));
+        int g0 = ch.first;
+        int64_t acc = 0;
+        int k = 1;
+        // cut k after a fraction (2k - 1) / (2n - 1) of the alignments
+        for (int g = ch.first; g < ch.second && k < n; ++g) {
+            acc += (int64_t)v.num_reads_per_group[g] * v.num_haps_per_group[g];
+            if (acc * (2 * n - 1) >= na * (2 * k - 1)) {
+                pieces.push_back({g0, g + 1, na});
+                g0 = g + 1;
+                ++k;
             }
-            if (g0 < ch.second)
-                pieces.push_back({g0, ch.second});
         }
-        chunks.swap(pieces);
+        if (g0 < ch.second)
+            pieces.push_back({g0, ch.second, na});
     }
+    // Alternate pieces run on a second call context and stream, each on a
+    // hardware queue of its own, so a piece's kernels start while the
+    // previous piece's last waves finish.  Without a free context (a server
+    // holds them) both slots use context 0 and the engine's stream.
+    int ctx2 = -1;
+    if (pieces.size() > 1) {
+        ctx2 = claim_ctx();
+        if (ctx2 > 0 && !stream2 && create_queue_stream(&stream2) != PHMM_OK) {
+            release_ctx(ctx2);
+            ctx2 = -1;
+        }
+    }
+    struct Release {
+        Engine* e;
+        int k;
+        ~Release()
+        {
+            if (k > 0)
+                e->release_ctx(k);
+        }
+    } release{this, ctx2};
     int rc;
     if ((rc = take_error_flag()))
         return rc;
-    int64_t out_off[2] = {0, 0};
+    int64_t out_off[kSlots] = {};
     auto drain = [&](int si) -> int {
         Slot& s = slots[si];
         if (!s.busy)
@@ -1263,20 +1310,37 @@ int Engine::score_batch(const phmm_batch_view& v, float* out)
         s.busy = false;
         return PHMM_OK;
     };
+    // Software pipeline over a ring of slots: piece k is staged and its
+    // copy enqueued before piece k-1's scoring is, so the copy engine always
+    // has the next piece queued while the kernels of the previous one run.
+    // Piece k scores on context/stream k % 2.
     int64_t next_off = 0;
-    for (size_t ci = 0; ci < chunks.size(); ++ci) {
-        const int si = (int)(ci % 2);
+    int pending = -1;  // staged piece whose scoring is not enqueued yet
+    for (size_t ci = 0; ci < pieces.size(); ++ci) {
+        const int si = (int)(ci % kSlots);
         if ((rc = drain(si)))
             return rc;
-        if ((rc = launch_chunk(slots[si], v, chunks[ci].first, chunks[ci].second)))
+        Slot& s = slots[si];
+        const bool alt = (ci % 2) == 1 && ctx2 > 0;
+        s.ctx = alt ? ctx2 : 0;
+        s.st = alt ? stream2 : stream;
+        s.eager = pieces.size() > 1;
+        const ChunkPart part{&v, pieces[ci].g0, pieces[ci].g1};
+        if ((rc = stage_parts(s, &part, 1, pieces[ci].table_n)))
             return rc;
         out_off[si] = next_off;
-        next_off += slots[si].n_scores;
+        next_off += s.n_scores;
+        if (pending >= 0 && (rc = compute_slot(slots[pending])))
+            return rc;
+        pending = si;
     }
-    for (int si = 0; si < 2; ++si)
+    if (pending >= 0 && (rc = compute_slot(slots[pending])))
+        return rc;
+    for (int si = 0; si < kSlots; ++si)
         if ((rc = drain(si)))
             return rc;
-    if ((rc = check(hipStreamSynchronize(stream), "final sync")))
+    if ((rc = check(hipStreamSynchronize(stream), "final sync")) ||
+        (ctx2 > 0 && (rc = check(hipStreamSynchronize(stream2), "final sync"))))
         return rc;
     finish_timing();
     if ((rc = take_error_flag()))

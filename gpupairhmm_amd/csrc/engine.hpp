@@ -30,6 +30,16 @@ struct Engine {
     phmm_options opt{};
     hipStream_t stream = nullptr;     // compute
     hipStream_t copy_stream = nullptr;
+    hipStream_t stream2 = nullptr;    // compute, host batches' alternate pieces
+    // the batch size that picks the tile table when >= 0 (the pieces of a
+    // host chunk score on the table of the whole chunk)
+    int64_t table_hint = -1;
+    // launch the kernels one by one even in graph mode: a graph launched
+    // behind a wait on another stream's event (a host piece's copy) holds
+    // the host thread in hipGraphLaunch until the copy has landed (1.5-2 ms
+    // per piece measured), which would stall the staging of the next piece
+    // (set for the pieces of a pipelined host batch)
+    bool eager_hint = false;
     std::vector<phmm_tile_config> tiles;
     // The one-template table of small batches (options.small_batch): the
     // classes of `tiles`, 16 rows per lane each (where 64 lanes hold them).
@@ -131,8 +141,14 @@ struct Engine {
         hipEvent_t h2d_done = nullptr, compute_done = nullptr;
         bool busy = false;
         int64_t n_scores = 0;
+        int ctx = 0;                   // call context and stream of the slot's piece
+        hipStream_t st = nullptr;
+        phmm_batch_view dv{};          // the staged batch on the device
+        int64_t table_n = -1;
+        bool eager = false;            // launch its kernels one by one (a pipelined piece)
     };
-    Slot slots[2];
+    static constexpr int kSlots = 3;
+    Slot slots[kSlots];
 
     ~Engine();
     int set_error(int code, const std::string& m)
@@ -160,6 +176,10 @@ struct Engine {
     // Bytes of each batch-sized scratch buffer for view v with NB read bins
     // and NW work bins, and the (NB, NW) of the table a call of v uses.
     static void scratch_sizes(const phmm_batch_view& v, int NB, int NW, size_t out[kNumScratch]);
+    // A stream on a hardware queue of its own (a CU mask naming every CU):
+    // streams otherwise share the process's few queues, where kernels of
+    // different streams run one after another.  Default priority.
+    int create_queue_stream(hipStream_t* out);
     bool range_rescue() const { return opt.range_rescue != 0 && opt.fp64_rescue && !roweps(); }
     void bins_for(const phmm_batch_view& v, int* NB, int* NW) const;
     // Device bytes context k holds (scratch, spill, counters).
@@ -185,16 +205,19 @@ struct Engine {
     int score_batch(const phmm_batch_view& v, float* out);
     int finish_timing();
     int take_error_flag();
-    // Stage groups [g0, g1) of host view v into slot s as a self-contained
-    // batch and enqueue copy-in, scoring and copy-out; s.compute_done fires
-    // when s.pin_scores holds the scores.
-    int launch_chunk(Slot& s, const phmm_batch_view& v, int g0, int g1);
     // Groups [g0, g1) of several views, concatenated in order into one slot.
     struct ChunkPart {
         const phmm_batch_view* v;
         int g0, g1;
     };
-    int launch_parts(Slot& s, const ChunkPart* parts, int nparts);
+    // Stage the parts into slot s as a self-contained batch and enqueue the
+    // copy-in (s.h2d_done fires when it landed); table_n: the batch size that
+    // picks the tile table (-1: the staged batch's own).
+    int stage_parts(Slot& s, const ChunkPart* parts, int nparts, int64_t table_n);
+    // Enqueue the scoring of a staged slot (its context and stream, behind
+    // the copy) and the copy-out (s.compute_done fires when s.pin_scores
+    // holds the scores).
+    int compute_slot(Slot& s);
     void free_slot(Slot& s);
 };
 

@@ -198,14 +198,8 @@ struct phmm_server {
 // Default priority, like every stream here.
 int phmm_server::create_launch_stream(hipStream_t* out)
 {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, eng->device) == hipSuccess && prop.multiProcessorCount > 0) {
-        std::vector<uint32_t> mask((size_t)(prop.multiProcessorCount + 31) / 32, 0xffffffffu);
-        if (hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data()) == hipSuccess)
-            return PHMM_OK;
-    }
-    (void)hipGetLastError();
-    return check(hipStreamCreateWithFlags(out, hipStreamNonBlocking), "stream");
+    const int rc = eng->create_queue_stream(out);
+    return rc ? phmm::fail(rc, std::string("server: ") + eng->err) : PHMM_OK;
 }
 
 // Allocates the launch contexts at the per-launch caps (engine scratch

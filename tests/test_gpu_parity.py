@@ -367,6 +367,27 @@ def test_device_api_matches_host_api(engine):
     assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
 
 
+@pytest.mark.parametrize("cfg,records", [(1, 10000), (2, 120)])
+def test_host_pieces_match_device(lib, cfg, records):
+    """A host batch of about 1e6 alignments is staged, copied and scored in
+    pieces on two call contexts and streams; every piece scores on the whole
+    batch's tile table, so the scores equal one device-resident call's bit
+    for bit (pageable and pinned host arrays alike)."""
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(cfg, seed=70 + cfg, num_records=records)
+    assert b.num_alignments > 750000  # four pieces
+    say(f"case: host pieces vs device call, config {cfg}, {b.num_alignments} alignments")
+    with G.Engine() as e:
+        db = e.upload(b)
+        dev = e.score_resident(db)
+        db.free()
+        host = e.score_batch(b)
+        pb = e.pinned_copy(b)
+        pinned = e.score_batch(pb)
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+    assert np.array_equal(pinned.view(np.uint32), dev.view(np.uint32))
+
+
 def _serve(srv, reqs, tags=None):
     """Submit every request, then collect all results: {tag: scores}."""
     tags = list(range(len(reqs))) if tags is None else tags
