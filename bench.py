@@ -307,7 +307,9 @@ def bench_config0(G, seed=1000, steps=20):
 
 def bench_serving(G, rates, seconds, pool_records, seed, parity_records=24):
     """Poisson arrivals of configuration-4 requests (one record each), and
-    parity of the served scores of a few pool records vs the FP64 oracle."""
+    parity of the served scores of a few pool records vs the FP64 oracle.
+    Each rate runs on a fresh server: 0.3 s of arrivals (not counted), then
+    `seconds` of arrivals whose latencies are reported."""
     pool = [G.HostBatch.synthetic(4, seed=seed + i, num_records=1) for i in range(pool_records)]
     served = {}
     out = []
@@ -328,10 +330,43 @@ def bench_serving(G, rates, seconds, pool_records, seed, parity_records=24):
         for rate in rates:
             srv = G.Server(eng, hbm_budget_bytes=64 << 30)
             rng = np.random.default_rng(seed + rate)
-            lat, done, cells_done, submitted, refused = [], 0, 0, 0, 0
-            t0 = time.perf_counter()
-            t_next = t0
-            i = 0
+
+            def arrivals(duration, take):
+                """Poisson arrivals at `rate` for `duration` seconds, then
+                drain; returns (submitted, refused, elapsed)."""
+                submitted = refused = got = 0
+                t0 = time.perf_counter()
+                t_next = t0
+                i = 0
+                while True:
+                    now = time.perf_counter()
+                    if now - t0 >= duration:
+                        break
+                    while t_next <= now:
+                        if srv.submit(pool[i % len(pool)], i):
+                            submitted += 1
+                        else:
+                            refused += 1
+                        i += 1
+                        t_next += rng.exponential(1.0 / rate)
+                    res = srv.poll(timeout_ms=0)
+                    while res is not None:
+                        take(res)
+                        got += 1
+                        res = srv.poll(timeout_ms=0)
+                while got < submitted:
+                    res = srv.poll(timeout_ms=10000)
+                    if res is None:
+                        break
+                    take(res)
+                    got += 1
+                return submitted, refused, time.perf_counter() - t0
+
+            # a short run at the same rate first (steady state: the staging
+            # ring and arena pages touched, every launch size seen)
+            arrivals(0.3, lambda res: None)
+            st0 = srv.stats()
+            lat, done, cells_done = [], 0, 0
 
             def take(res):
                 nonlocal done, cells_done
@@ -342,29 +377,9 @@ def bench_serving(G, rates, seconds, pool_records, seed, parity_records=24):
                 if k < parity_records and k not in served:
                     served[k] = sc
                 done += 1
-            while True:
-                now = time.perf_counter()
-                if now - t0 >= seconds:
-                    break
-                while t_next <= now:
-                    req = pool[i % len(pool)]
-                    if srv.submit(req, i):
-                        submitted += 1
-                    else:
-                        refused += 1
-                    i += 1
-                    t_next += rng.exponential(1.0 / rate)
-                res = srv.poll(timeout_ms=0)
-                while res is not None:
-                    take(res)
-                    res = srv.poll(timeout_ms=0)
-            while done < submitted:
-                res = srv.poll(timeout_ms=10000)
-                if res is None:
-                    break
-                take(res)
-            elapsed = time.perf_counter() - t0
+            submitted, refused, elapsed = arrivals(seconds, take)
             st = srv.stats()
+            st["launches"] -= st0["launches"]
             srv.close()
             lat_ms = np.array(lat) * 1e3
             out.append({"rate_req_s": rate, "submitted": submitted, "refused": refused,

@@ -132,7 +132,8 @@ class ServerOptions(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
         ("max_batch_requests", ctypes.c_int32),
         ("max_batch_alignments", ctypes.c_int64), ("max_batch_bytes", ctypes.c_int64),
-        ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("streams", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("latency_batch_alignments", ctypes.c_int64),
     ]
 
 
@@ -702,7 +703,7 @@ class Server:
     def __init__(self, engine: Engine, hbm_budget_bytes: int = 64 << 30,
                  max_inflight: int = 1 << 20, max_batch_requests: int = 4096,
                  max_batch_alignments: int | None = None, max_batch_bytes: int | None = None,
-                 streams: int | None = None):
+                 streams: int | None = None, latency_batch_alignments: int | None = None):
         lib = load_library()
         o = ServerOptions()
         lib.phmm_server_options_default(ctypes.byref(o))
@@ -715,6 +716,8 @@ class Server:
             o.max_batch_bytes = int(max_batch_bytes)
         if streams is not None:
             o.streams = int(streams)
+        if latency_batch_alignments is not None:
+            o.latency_batch_alignments = int(latency_batch_alignments)
         h = ctypes.c_void_p()
         _check(lib.phmm_server_create(engine._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -484,7 +484,10 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     fp.retry_items = static_cast<WorkItem*>(sp[kSuperTasks]);
     fp.retry_counts = bp.retry_counts;
     fp.rescue_exps = static_cast<uint8_t*>(sp[kRescueExps]);
-    fp.re_on = range_rescue() ? 1 : 0;
+    // the range-extended pass pays off where the rescue is throughput-bound
+    // (the tuned table); a small batch's rescue costs one sweep's latency per
+    // stage, so a second stage would only add one
+    fp.re_on = range_rescue() && act != &tiles_small ? 1 : 0;
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;

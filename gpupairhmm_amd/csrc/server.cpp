@@ -167,6 +167,7 @@ struct phmm_server {
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Request> queue;  // admitted, not launched
+    int64_t queued_na = 0;      // alignments in the queue
     std::deque<Result> done;
     std::vector<LaunchCtx> ctxs;
     bool stop = false;
@@ -511,17 +512,24 @@ void phmm_server::run()
             if (stop && queue.empty() && !any_busy)
                 return;
             if (freec && !queue.empty()) {
-                // merge waiting requests in arrival order, up to the caps
+                // merge waiting requests in arrival order, up to the caps:
+                // the latency cap unless the backlog says the server is
+                // overloaded
+                const int64_t cap =
+                    opt.latency_batch_alignments > 0 && queued_na < 4 * opt.max_batch_alignments
+                        ? std::min(opt.latency_batch_alignments, opt.max_batch_alignments)
+                        : opt.max_batch_alignments;
                 int64_t na = 0, bytes = 0, groups = 0;
                 while (!queue.empty() && (int)batch.size() < opt.max_batch_requests) {
                     const Request& r = queue.front();
-                    if (!batch.empty() && (na + r.nA > opt.max_batch_alignments ||
+                    if (!batch.empty() && (na + r.nA > cap ||
                                            bytes + (int64_t)r.bytes > opt.max_batch_bytes ||
                                            groups + r.G > max_groups))
                         break;
                     na += r.nA;
                     bytes += (int64_t)r.bytes;
                     groups += r.G;
+                    queued_na -= r.nA;
                     batch.push_back(std::move(queue.front()));
                     queue.pop_front();
                 }
@@ -567,6 +575,7 @@ extern "C" void phmm_server_options_default(phmm_server_options* o)
     o->max_batch_alignments = 1 << 20;
     o->max_batch_bytes = (int64_t)64 << 20;
     o->streams = 2;
+    o->latency_batch_alignments = 1 << 19;
 }
 
 extern "C" int phmm_server_create(phmm_engine* e, const phmm_server_options* opt,
@@ -713,6 +722,7 @@ extern "C" int phmm_server_submit(phmm_server* s, const phmm_batch_view* req, ui
     r.t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
+        s->queued_na += r.nA;
         s->queue.push_back(std::move(r));
         s->st.submitted += 1;
         s->st.queued = (int64_t)s->queue.size();

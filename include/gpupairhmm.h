@@ -89,26 +89,27 @@ typedef struct phmm_options {
                                   graph; 0 launches the kernels one by one */
     int32_t small_batch;       /* batches of at most this many alignments
                                   (either mode) use a one-template tile table
-                                  (16 rows per lane for every class), so their
-                                  latency is one launch per stage rather than
-                                  one per tile template.  Scores are
-                                  bit-identical across tables except where the
-                                  range-extended pass accepts an alignment on
-                                  one table and hands it to FP64 on the other
-                                  (both within 5e-5 log10 of the FP64
-                                  oracle).  0: always the tuned table.
-                                  Default 1 << 18. */
+                                  (16 rows per lane for every class) and
+                                  rescue in FP64 directly, so their latency
+                                  is one launch per stage rather than one per
+                                  tile template.  Scores are bit-identical
+                                  across tables above the rescue threshold;
+                                  below it the tuned table's range-extended
+                                  results agree with FP64 to FP32 accuracy
+                                  (within 5e-5 log10).  0: always the tuned
+                                  table.  Default 1 << 18. */
     int32_t f64_rows;          /* rows per lane of every FP64 rescue tile
                                   (4, 6, 8 or 10; SAFE tiles at most 8); 0
                                   (default): chosen per bin from the
                                   measured FP64 tile rates */
-    int32_t range_rescue;      /* 1 (default): results below rescue_min_sum
-                                  are first recomputed in FP32 with a power-
-                                  of-two scale per tile lane, chosen from the
-                                  first pass's per-lane maxima; only what
-                                  that pass cannot accept goes to FP64.
-                                  0: every such result is recomputed in FP64.
-                                  Not used under PHMM_SEM_GCP. */
+    int32_t range_rescue;      /* 1 (default): on the tuned table, results
+                                  below rescue_min_sum are first recomputed
+                                  in FP32 with a power-of-two scale per tile
+                                  lane, chosen from the first pass's per-lane
+                                  magnitudes; only what that pass cannot
+                                  accept goes to FP64.  0: every such result
+                                  is recomputed in FP64.  Not used under
+                                  PHMM_SEM_GCP or on the small-batch table. */
     int32_t reserved[2];
 } phmm_options;
 
@@ -311,7 +312,16 @@ typedef struct phmm_server_options {
                                   + hap bytes + index arrays); a larger
                                   request is refused with PHMM_ERANGE */
     int32_t streams;           /* launch contexts (streams), 1..3; default 2 */
-    int32_t reserved[3];
+    int32_t reserved0;
+    int64_t latency_batch_alignments; /* alignments per merged launch while
+                                  the queue holds less than 4 x
+                                  max_batch_alignments (a launch above the
+                                  engine's small_batch runs the tuned table's
+                                  longer sequence, and requests arriving
+                                  behind it wait for it); beyond that backlog
+                                  launches grow to max_batch_alignments for
+                                  throughput.  <= 0: max_batch_alignments.
+                                  Default 1 << 19. */
 } phmm_server_options;
 
 void phmm_server_options_default(phmm_server_options* opt);

@@ -27,16 +27,15 @@ def _max_delta(a, b):
 
 
 @pytest.mark.parametrize("cfg,seed,records", [(1, 1001, 60), (2, 1002, 40), (3, 1003, 3)])
-@pytest.mark.parametrize("small_batch", [None, 0])
-def test_range_rescue_agrees_with_fp64(lib, oracle, cfg, seed, records, small_batch):
+def test_range_rescue_agrees_with_fp64(lib, oracle, cfg, seed, records):
     import gpupairhmm_amd as G
     b = G.HostBatch.synthetic(cfg, seed=seed, num_records=records)
     say(f"case: range-extended rescue, config {cfg}, {records} records, "
-        f"{b.num_alignments} alignments, small_batch={small_batch}")
-    with G.Engine(small_batch=small_batch) as on:
+        f"{b.num_alignments} alignments")
+    with G.Engine(small_batch=0) as on:  # the tuned table at any size
         got = on.score_batch(b)
         st = on.stats()
-    with G.Engine(small_batch=small_batch, range_rescue=False) as off:
+    with G.Engine(small_batch=0, range_rescue=False) as off:
         f64 = off.score_batch(b)
         sf = off.stats()
     assert st.rescued == sf.rescued > 0
@@ -82,7 +81,7 @@ def test_range_rescue_every_alignment(lib, oracle):
     b = G.HostBatch.synthetic(1, seed=11, num_records=40)
     n = b.num_alignments
     say(f"case: every alignment rescued, {n} alignments")
-    with G.Engine(rescue_min_sum=float("inf")) as e:
+    with G.Engine(rescue_min_sum=float("inf"), small_batch=0) as e:
         got = e.score_batch(b)
         st = e.stats()
     assert st.rescued == n
@@ -118,3 +117,20 @@ def test_range_rescue_long_reads(lib, oracle):
     assert st.rescued == b.num_alignments
     assert st.f64_alignments < st.rescued
     assert _max_delta(got, ref) <= FP32_TOL
+
+
+def test_small_batches_rescue_in_fp64(lib):
+    """Below small_batch the rescue goes to FP64 directly (its time is one
+    sweep's latency per stage); above it the range-extended pass takes most."""
+    import gpupairhmm_amd as G
+    b = G.HostBatch.synthetic(2, seed=1002, num_records=6)
+    say(f"case: small batch rescue, {b.num_alignments} alignments")
+    with G.Engine() as e:
+        e.score_batch(b)
+        small = e.stats()
+    with G.Engine(small_batch=0) as e:
+        e.score_batch(b)
+        tuned = e.stats()
+    assert small.rescued == tuned.rescued > 0
+    assert small.f64_alignments == small.rescued
+    assert tuned.f64_alignments < tuned.rescued
