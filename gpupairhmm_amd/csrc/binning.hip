@@ -932,22 +932,35 @@ __global__ __launch_bounds__(64 * NW) void k_scatter_t(BinParams bp)
     }
 }
 
-__global__ void k_add_count(int64_t* total, const int32_t* counts, int n)
+__global__ void k_add_counts(CountSums c)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int64_t s = 0;
-        for (int i = 0; i < n; ++i)
-            s += counts[i];
-        // calls on several streams may add at once
-        atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)s);
+        for (int k = 0; k < c.n; ++k) {
+            int64_t s = 0;
+            for (int i = 0; i < c.len[k]; ++i)
+                s += c.counts[k][i];
+            // calls on several streams may add at once
+            atomicAdd(reinterpret_cast<unsigned long long*>(c.total[k]), (unsigned long long)s);
+        }
     }
 }
 
-// *total += sum of counts[0..n)
+hipError_t launch_add_counts(const CountSums& c, hipStream_t st)
+{
+    if (c.n < 1 || c.n > CountSums::kMax)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_add_counts, dim3(1), dim3(64), 0, st, c);
+    return hipGetLastError();
+}
+
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, st, total, counts, n);
-    return hipGetLastError();
+    CountSums c;
+    c.n = 1;
+    c.total[0] = total;
+    c.counts[0] = counts;
+    c.len[0] = n;
+    return launch_add_counts(c, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -708,9 +708,13 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
     // list holds, at this point, the one-wave FP32 kernel's results that
     // need FP64 (the FP64 tile pass adds to it later).
     int64_t* tot = d_rescued_total;
-    if ((rc = check(launch_add_count(tot, fp.final_count, 1, st), "rescue count")) ||
-        (rc = check(launch_add_count(tot + 1, fp.final_count, 1, st), "rescue count")))
-        return rc;
+    {
+        CountSums c;
+        c.add(tot, fp.final_count, 1);
+        c.add(tot + 1, fp.final_count, 1);
+        if ((rc = check(launch_add_counts(c, st), "rescue count")))
+            return rc;
+    }
     if (fp.re_on) {
         int nre = 0;
         for (int b0 = 2 * ntiles - 1; b0 >= 0; --b0) {
@@ -859,12 +863,15 @@ int Engine::enqueue_rescue(const FwdParams& fp, const BinParams& bp, bool plain,
         return rc;
     // statistics: the rescue lists; in FP64: the retry lists and the rescue
     // lists of bins without a range-extended pass
-    if ((rc = check(launch_add_count(tot, fp.rescue_counts, 2 * ntiles, st), "rescue count")))
-        return rc;
+    {
+        CountSums c;
+        c.add(tot, fp.rescue_counts, 2 * ntiles);
+        c.add(tot + 1, fp.re_on ? fp.retry_counts : fp.rescue_counts, 2 * ntiles);
+        if ((rc = check(launch_add_counts(c, st), "rescue count")))
+            return rc;
+    }
     if (!fp.re_on)
-        return check(launch_add_count(tot + 1, fp.rescue_counts, 2 * ntiles, st), "rescue count");
-    if ((rc = check(launch_add_count(tot + 1, fp.retry_counts, 2 * ntiles, st), "rescue count")))
-        return rc;
+        return PHMM_OK;
     for (int b = 0; b < 2 * ntiles; ++b)  // tables with rows lacking the range-extended pass
         if (!re_bin(b) &&
             (rc = check(launch_add_count(tot + 1, fp.rescue_counts + b, 1, st), "rescue count")))

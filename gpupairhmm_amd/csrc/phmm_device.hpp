@@ -243,7 +243,24 @@ struct BinParams {
 // grids are sized to them.
 hipError_t run_binning(const BinParams& bp, const BatchDesc& hd, bool fixed_grids, bool small,
                        hipStream_t st);
+// *total += sum of counts[0..n)
 hipError_t launch_add_count(int64_t* total, const int32_t* counts, int n, hipStream_t st);
+// Several such sums in one launch.
+struct CountSums {
+    static constexpr int kMax = 4;
+    int32_t n = 0;
+    int64_t* total[kMax] = {};
+    const int32_t* counts[kMax] = {};
+    int32_t len[kMax] = {};
+    void add(int64_t* t, const int32_t* c, int l)
+    {
+        total[n] = t;
+        counts[n] = c;
+        len[n] = l;
+        ++n;
+    }
+};
+hipError_t launch_add_counts(const CountSums& c, hipStream_t st);
 // Several bins in one persistent launch.  Bin bin[i] holds count_i =
 // counts[count_stride*bin[i]] units starting at begin_i = range ?
 // range[2*bin[i]] : 0, both read on the device.  A unit is a SuperTask
