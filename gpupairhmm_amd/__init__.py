@@ -109,7 +109,7 @@ class Stats(ctypes.Structure):
         ("engine_seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
         ("graph_launches", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
         ("rescue_seconds", ctypes.c_double), ("f64_alignments", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 4),
+        ("scaled_alignments", ctypes.c_int64), ("reserved", ctypes.c_int64 * 3),
     ]
 
 
@@ -486,6 +486,7 @@ class EngineStats:
     graph_captures: int = 0
     rescue_seconds: float = 0.0
     f64_alignments: int = 0
+    scaled_alignments: int = 0
 
 
 class _PinnedBlocks:
@@ -531,7 +532,7 @@ class Engine:
                  transfer_batchsize: int = 1000000, fp64_rescue: bool = True,
                  rescue_min_sum: float = FLT_MIN, time_kernels: bool = False,
                  graphs: bool = True, small_batch: int | None = None, f64_rows: int = 0,
-                 range_rescue: bool = True):
+                 range_rescue: int | bool = True):
         lib = load_library()
         o = Options()
         lib.phmm_options_default(ctypes.byref(o))
@@ -545,7 +546,9 @@ class Engine:
         o.time_kernels = 1 if time_kernels else 0
         o.graphs = 1 if graphs else 0
         o.f64_rows = int(f64_rows)
-        o.range_rescue = 1 if range_rescue else 0
+        # True / 1: per-lane scaling inside the FP32 pass; 2: a second,
+        # range-extended FP32 pass; False / 0: FP64 only
+        o.range_rescue = int(range_rescue)
         h = ctypes.c_void_p()
         _check(lib.phmm_engine_create(int(device), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -680,7 +683,7 @@ class Engine:
         return EngineStats(s.alignments, s.cells, s.rescued, s.fast_alignments,
                            s.multitile_alignments, s.engine_seconds, s.kernel_seconds,
                            s.graph_launches, s.graph_captures, s.rescue_seconds,
-                           s.f64_alignments)
+                           s.f64_alignments, s.scaled_alignments)
 
     def reset_stats(self) -> None:
         self._chk(load_library().phmm_engine_reset_stats(self._h))

@@ -488,6 +488,7 @@ int Engine::score_device(const phmm_batch_view& v, float* d_scores, hipStream_t 
     // (the tuned table); a small batch's rescue costs one sweep's latency per
     // stage, so a second stage would only add one
     fp.re_on = range_rescue() && act != &tiles_small ? 1 : 0;
+    fp.scaled_total = reinterpret_cast<unsigned long long*>(d_rescued_total + 2);
     fp.rescue_min = opt.rescue_min_sum;
     fp.rescue_on = opt.fp64_rescue;
     fp.semantics = opt.semantics;
@@ -622,11 +623,16 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             if (ctr >= kTaskCounters - kF64Sets - kReSets)
                 return set_error(PHMM_ERANGE, "too many set launches");
             bs.next_task = bp.task_ctr + ctr++;
+            const bool scaled = scaled_pass() && tile_re_supported(tc.rows);
             const int grid = std::min(
-                tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false, roweps()));
+                tasks, num_cus * tile_set_blocks_per_cu(tc.rows, (b % 2) == 0, false, roweps(),
+                                                        scaled));
             FwdParams fpl = fp;
             fpl.re_on = fp.re_on && tile_re_supported(tc.rows);
-            if ((rc = check(launch_fwd_tile_set(tc.rows, (b % 2) == 0, fpl, bs, grid, st, roweps()),
+            if ((rc = check(scaled ? launch_fwd_tile_set_scaled(tc.rows, (b % 2) == 0, fpl, bs,
+                                                                grid, st)
+                                   : launch_fwd_tile_set(tc.rows, (b % 2) == 0, fpl, bs, grid,
+                                                         st, roweps()),
                             "forward kernel")))
                 return rc;
         }
@@ -656,7 +662,9 @@ int Engine::enqueue_fwd(const FwdParams& fp, const BinParams& bp, bool plain, hi
             bs.next_task = bp.task_ctr + ctr++;
             FwdParams fpl = fp;
             fpl.re_on = fp.re_on && tile_re_supported(rows);
-            if ((rc = check(launch_fwd_tile_set(rows, fast, fpl, bs, 0, st, roweps()),
+            const bool scaled = scaled_pass() && tile_re_supported(rows);
+            if ((rc = check(scaled ? launch_fwd_tile_set_scaled(rows, fast, fpl, bs, 0, st)
+                                   : launch_fwd_tile_set(rows, fast, fpl, bs, 0, st, roweps()),
                             "forward kernel")))
                 return rc;
         }
@@ -1423,6 +1431,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: unknown semantics");
     if (o.f64_rows != 0 && o.f64_rows != 4 && o.f64_rows != 6 && o.f64_rows != 8 && o.f64_rows != 10)
         return phmm::fail(PHMM_EINVAL, "phmm_engine_create: f64_rows must be 0, 4, 6, 8 or 10");
+    if (o.range_rescue < 0 || o.range_rescue > 2)
+        return phmm::fail(PHMM_EINVAL, "phmm_engine_create: range_rescue must be 0, 1 or 2");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         hipGetLastError();
@@ -1454,8 +1464,8 @@ extern "C" int phmm_engine_create(int device, const phmm_options* opt, phmm_engi
     if ((rc = e.check(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking), "stream")) ||
         (rc = e.check(hipStreamCreateWithFlags(&e.copy_stream, hipStreamNonBlocking), "stream")) ||
         (rc = e.check(hipMalloc(&e.d_ph2pr, 128 * sizeof(float)), "ph2pr")) ||
-        (rc = e.check(hipMalloc(&e.d_rescued_total, 2 * sizeof(int64_t)), "counter")) ||
-        (rc = e.check(hipMemset(e.d_rescued_total, 0, 2 * sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipMalloc(&e.d_rescued_total, Engine::kTotals * sizeof(int64_t)), "counter")) ||
+        (rc = e.check(hipMemset(e.d_rescued_total, 0, Engine::kTotals * sizeof(int64_t)), "counter")) ||
         (rc = e.check(hipHostMalloc(&e.h_small, 256, hipHostMallocDefault), "pinned")) ||
         (rc = e.init_ctx(0)) || (rc = phmm::noalloc_init(device)))
         return bail(rc);
@@ -1485,7 +1495,7 @@ extern "C" int phmm_engine_stats(const phmm_engine* h, phmm_stats* out)
     hipSetDevice(m->e.device);
     hipStreamSynchronize(m->e.stream);
     m->e.finish_timing();
-    int64_t rescued[2] = {0, 0};
+    int64_t rescued[phmm::Engine::kTotals] = {0, 0, 0};
     if (m->e.d_rescued_total) {
         if (hipMemcpy(rescued, m->e.d_rescued_total, sizeof rescued, hipMemcpyDeviceToHost) !=
             hipSuccess)
@@ -1494,6 +1504,7 @@ extern "C" int phmm_engine_stats(const phmm_engine* h, phmm_stats* out)
     *out = h->e.stats;
     out->rescued = rescued[0];
     out->f64_alignments = rescued[1];
+    out->scaled_alignments = rescued[2];
     out->graph_launches = h->e.graph_launches;
     out->graph_captures = h->e.graph_captures;
     return PHMM_OK;
@@ -1511,7 +1522,7 @@ extern "C" int phmm_engine_reset_stats(phmm_engine* h)
     if (h->e.d_rescued_total) {
         hipSetDevice(h->e.device);
         hipDeviceSynchronize();
-        if (hipMemset(h->e.d_rescued_total, 0, 2 * sizeof(int64_t)) != hipSuccess)
+        if (hipMemset(h->e.d_rescued_total, 0, phmm::Engine::kTotals * sizeof(int64_t)) != hipSuccess)
             hipGetLastError();
     }
     return PHMM_OK;

@@ -128,7 +128,9 @@ struct Engine {
     };
     std::vector<StageEvents> ev_free, ev_pending;
 
-    int64_t* d_rescued_total = nullptr;  // cumulative [rescued, of them in FP64]
+    // cumulative [rescued, of them in FP64, finished by in-pass scaling]
+    int64_t* d_rescued_total = nullptr;
+    static constexpr int kTotals = 3;
 
     hipStream_t cap_stream = nullptr;
     int64_t graph_captures = 0, graph_launches = 0;
@@ -180,7 +182,10 @@ struct Engine {
     // streams otherwise share the process's few queues, where kernels of
     // different streams run one after another.  Default priority.
     int create_queue_stream(hipStream_t* out);
-    bool range_rescue() const { return opt.range_rescue != 0 && opt.fp64_rescue && !roweps(); }
+    // options.range_rescue 1: the FP32 tiles scale per lane in the pass
+    // itself; 2: a second, range-extended FP32 pass over the rescue lists
+    bool scaled_pass() const { return opt.range_rescue == 1 && opt.fp64_rescue && !roweps(); }
+    bool range_rescue() const { return opt.range_rescue == 2 && opt.fp64_rescue && !roweps(); }
     void bins_for(const phmm_batch_view& v, int* NB, int* NW) const;
     // Device bytes context k holds (scratch, spill, counters).
     size_t ctx_bytes(int k) const;

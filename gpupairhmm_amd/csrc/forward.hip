@@ -357,11 +357,14 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
 // sum over the columns of its last row's outgoing chain -- at no cost in
 // the sweep.  The first pass keeps its ilogb for the recompute's scales,
 // the recompute checks them with it.
-template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false>
-__device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
-                                           const TileRead<T, NR, ROWEPS>& R, bool active, int G,
-                                           int t, int HL, int hoff, const WorkItem& it,
-                                           int rescue_base, int bin)
+// Returns, for the scaled instances (DS), the number of the wave's
+// alignments that fell below the rescue threshold and were finished by the
+// scaling (uniform across the wave); 0 otherwise.
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false>
+__device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
+                                          const TileRead<T, NR, ROWEPS>& R, bool active, int G,
+                                          int t, int HL, int hoff, const WorkItem& it,
+                                          int rescue_base, int bin)
 {
     const BatchDesc& bd = *p.bd;
     using TT = TileTraits<T>;
@@ -373,8 +376,10 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
     const T beta = R.beta;
     const int L = R.L;
     const float nl = R.nl;
-    constexpr bool REC = F32 && !ROWEPS && re_rows(NR);  // record per-lane magnitudes
+    constexpr bool REC = F32 && !ROWEPS && re_rows(NR) && !DS;  // record per-lane magnitudes
     static_assert(!RE || REC, "the range-extended pass is an FP32 pass of the one-eps tiles");
+    static_assert(!DS || (F32 && !ROWEPS && !RE && re_rows(NR)),
+                  "in-pass scaling: FP32 one-eps tiles of the range-extended rows");
     // the group's lane that forms the sum, the group's first lane and its
     // lanes as a mask (computed where used: nothing extra stays live in
     // the sweep)
@@ -442,6 +447,59 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
         if (!first)
             nf = ldexpf(1.0f, re_dE < -126 ? -126 : re_dE > 126 ? 126 : re_dE);
         C0 = ldexpf((float)C0, -kReShift);
+    }
+    int ds_E = 0;  // DS: this lane's exponent (held = true * 2^ds_E)
+    bool ds_any = false;  // DS: some lane of the wave is scaled (uniform)
+    // DS: x capped at kDsSpan per lane above the group's lanes over it
+    // (x_t <= x_k + kDsSpan * (t - k) for k < t: a prefix minimum of
+    // x - kDsSpan * t)
+    auto ds_span_cap = [&](int x) {
+        int z = x - kDsSpan * t;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(z, o, 64);
+            if (t >= o)
+                z = y < z ? y : z;
+        }
+        return z + kDsSpan * t;
+    };
+    // DS: the lane above's value of an exponent.  The shifted value is
+    // kept opaque to the compiler: fused into the subtraction that uses it
+    // (v_subrev_u32_dpp), it measured as E_{t-1} - E_t on gfx950.
+    auto ds_above = [&](int v) {
+        int u = shr1i(v);
+        asm volatile("" : "+v"(u));
+        return u;
+    };
+    // DS: the transfer factor 2^(E_t - E_{t-1}), exact as long as the
+    // exponents stay within kDsSpan
+    auto ds_factor = [&]() -> T {
+        const int d = ds_E - ds_above(ds_E);
+        return first ? (T)0 : (T)ldexpf(1.0f, d);
+    };
+    if constexpr (DS) {
+        // log2 of the insertion floor's factor over this lane's rows (the
+        // chain coefficient of each row; FAST row 0 has none), summed over
+        // the lanes above within the group
+        float lg = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const float e = FAST ? (float)R.c2[j] : (float)R.c3[j];
+            lg += e > 0.0f ? __log2f(e) : 0.0f;
+        }
+        float incl = lg;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float y = __shfl_up(incl, o, 64);
+            if (t >= o)
+                incl += y;
+        }
+        // the floor of the lane's last row at 2^kDsFloor
+        const float fl = (active && C0 > 0) ? __log2f((float)C0) + incl : 0.0f;
+        const int e0 = (int)floorf((float)kDsFloor - fl);
+        ds_E = ds_span_cap(first || e0 < 0 ? 0 : e0);
+        nf = ds_factor();
+        ds_any = __ballot(ds_E > 0) != 0;
     }
     const T c0f = first ? C0 : (T)0;
     // M and D of the row above reach a lane two steps after they were
@@ -567,11 +625,62 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
         whi = wnx;
         wnx = hap_dword(((s4 - t) >> 2) + 3);  // arithmetic shift: floor
     };
+    // DS: every kDsEvery steps a scaled lane whose incoming values passed
+    // 2^kDsGuard scales down to put them at 2^kDsGuardTo (never below
+    // E = 0), rescaling what it holds (the values already taken from the
+    // lane above included) and its transfer factor.  Called at wave-uniform
+    // points; every lane of the wave takes part in the exchange.
+    auto ds_check = [&]() {
+        // values are never negative: no absolute values needed
+        const float m = fmaxf(fmaxf(fmaxf((float)Mr[0], (float)Mr[1]), fmaxf((float)Dr[0], (float)Dr[1])),
+                              (float)Iin);
+        int x = ds_E;
+        if (ds_E > 0 && m > kDsGuardF) {
+            // v_frexp_exp reads inf as 0: an overflowed lane keeps its
+            // exponent and fails acceptance
+            const int g = ds_E + kDsGuardTo - (__builtin_amdgcn_frexp_expf(m) - 1);
+            x = g < 0 ? 0 : g;
+        }
+        // any distance in one step: v_ldexp is exact wherever the result
+        // stays normal
+        if (__ballot(x != ds_E)) {
+            // no lane more than kDsSpan above the lane over it, so the
+            // transfer factor stays a normal float (a lane whose upper rows
+            // meet a read's alignment scales down before its last row, which
+            // the lane below sees, does); the scan only where a gap opened
+            const int xu = ds_above(x);
+            if (__ballot(!first && x - xu > kDsSpan))
+                x = ds_span_cap(x);
+            const int S = x - ds_E;
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                M[j] = ldexpf(M[j], S);
+                D[j] = ldexpf(D[j], S);
+            }
+            Mr[0] = ldexpf(Mr[0], S);
+            Mr[1] = ldexpf(Mr[1], S);
+            Dr[0] = ldexpf(Dr[0], S);
+            Dr[1] = ldexpf(Dr[1], S);
+            Iin = ldexpf(Iin, S);
+            acc = ldexpf(acc, S);
+            ds_E = x;
+            nf = ds_factor();
+            ds_any = __ballot(ds_E > 0) != 0;
+        }
+    };
     int s4 = 0;
-    for (; s4 + 4 <= n_free && s4 < nsteps; s4 += 4)
+    for (; s4 + 4 <= n_free && s4 < nsteps; s4 += 4) {
         sweep4(s4, false);
-    for (; s4 < nsteps; s4 += 4)
+        if constexpr (DS)
+            if (ds_any && (s4 & (kDsEvery - 4)) == kDsEvery - 4)
+                ds_check();
+    }
+    for (; s4 < nsteps; s4 += 4) {
         sweep4(s4, true);
+        if constexpr (DS)
+            if (ds_any && (s4 & (kDsEvery - 4)) == kDsEvery - 4)
+                ds_check();
+    }
 
     if constexpr (RE) {
         // each lane's incoming magnitude, in its own scale: the lane above's
@@ -606,11 +715,58 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
                 p.retry_items[rescue_base + n] = it;
             }
         }
-        return;
+        return 0;
+    }
+    // DS acceptance: every scaled lane up to the result lane holds a finite
+    // running sum (an unscaled lane's sum can overflow harmlessly in the
+    // FAST form's upper rows; an overflowed value reaches the result lane's
+    // sum), and neighbouring exponents keep the transfer factor a normal
+    // float.  Nothing that mattered was flushed: a lane starts with its
+    // insertion floor in range and scales down only by what its largest
+    // incoming value grew past 2^kDsGuard.
+    unsigned long long ds_fb = 0;
+    bool ds_done = false;  // finished by the scaling below the rescue threshold
+    if constexpr (DS) {
+        const int res_t = res_lane();
+        const int dl = ds_E - ds_above(ds_E);
+        bool bad = false;
+        if (active && t <= res_t)
+            bad = (ds_E > 0 && !((float)acc <= 3.4028234663852886e38f)) ||
+                  (t >= 1 && (dl > 127 || dl < -126));
+        ds_fb = __ballot(bad) & group_mask();
     }
     int slot = -1;  // REC: the rescued alignment's rescue-list slot
     if (active && t == L / NR) {
-        if (F32) {
+        if (DS && (ds_fb || ds_E > 0 || !((float)acc <= 3.4028234663852886e38f))) {
+            const float a = (float)acc;
+            if (!ds_fb && a >= 0x1p-100f && a <= 3.4028234663852886e38f) {
+                // the unscaled sum, where it is a normal float: the score and
+                // threshold test of the unscaled instance, bit for bit
+                const float at = ldexpf(a, -ds_E);
+                float raw;
+                if (at >= 0x1p-126f) {
+                    bd.scores[it.out] = FAST ? log10f(at) - (kLog10Scale - kLog10FastKappa)
+                                             : log10f(at) - (kLog10Scale + log10f((float)beta));
+                    raw = FAST ? at * nl : at / (float)beta;
+                } else {
+                    // a * 2^-E = m * 2^K exactly: the score does not depend
+                    // on how the rows were split into lanes
+                    int e2 = 0;
+                    const float m = frexpf(a, &e2);
+                    const int K = e2 - ds_E;
+                    const double off = FAST ? (double)kLog10Scale - (double)kLog10FastKappa
+                                            : (double)kLog10Scale + log10((double)(float)beta);
+                    bd.scores[it.out] =
+                        (float)(log10((double)m) + (double)K * 0.30102999566398119521 - off);
+                    raw = ldexpf(FAST ? a * nl : a / (float)beta, -ds_E);
+                }
+                ds_done = !(raw >= p.rescue_min);
+            } else {
+                bd.scores[it.out] = -INFINITY;
+                const int n = atomicAdd(p.rescue_counts + bin, 1);
+                p.rescue_items[rescue_base + n] = it;
+            }
+        } else if (F32) {
             const float a = (float)acc;
             float score, raw;
             if (FAST) {
@@ -652,12 +808,15 @@ __device__ __forceinline__ void tile_sweep(const FwdParams& p, const typename Ti
                 (uint8_t)(int8_t)(e < -128 ? -128 : e > 127 ? 127 : e);
         }
     }
+    if constexpr (DS)
+        return __popcll(__ballot(ds_done));
+    return 0;
 }
 
 // A task of 64/G consecutive alignments of an item list (the FP64 recompute
 // of rescue lists): every group sets up its own read and sweeps its own hap.
-template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false>
-__device__ __forceinline__ void tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false>
+__device__ __forceinline__ int tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const WorkItem* items, int begin, int count, int G,
                                                 int task, int rescue_base, int bin)
 {
@@ -682,16 +841,17 @@ __device__ __forceinline__ void tile_items_task(const FwdParams& p, typename Til
     tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, it.read, t, R);
     const int HL = active ? bd.hap_len[it.hap] : 0;
     const int hoff = active ? bd.hap_off[it.hap] : 0;
-    tile_sweep<T, NR, FAST, ROWEPS, RE>(p, tab, R, active, G, t, HL, hoff, it, rescue_base, bin);
+    return tile_sweep<T, NR, FAST, ROWEPS, RE, DS>(p, tab, R, active, G, t, HL, hoff, it,
+                                                   rescue_base, bin);
 }
 
 // A super-task: the groups set up their reads once and sweep the segment's
 // haps in turn (all groups share each hap, so every sweep is uniform across
 // the wave).  Empty haps were scored -inf by the scatter; haps holding a
 // byte other than A/C/G/T belong to the exact bin.
-template <typename T, int NR, bool FAST, bool ROWEPS>
-__device__ __forceinline__ void tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
-                                                const SuperTask& st, int G, int rescue_base, int bin)
+template <typename T, int NR, bool FAST, bool ROWEPS, bool DS = false>
+__device__ __forceinline__ int tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
+                                               const SuperTask& st, int G, int rescue_base, int bin)
 {
     const BatchDesc& bd = *p.bd;
     const int lane = threadIdx.x;
@@ -702,6 +862,7 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
     const int rd = active ? p.read_perm[st.perm + grp] : 0;
     TileRead<T, NR, ROWEPS> R;
     tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, rd, t, R);
+    int done = 0;
     for (int h = st.hs; h < st.he; ++h) {
         const int hap = st.h0 + h;
         const int HL = bd.hap_len[hap];
@@ -712,9 +873,10 @@ __device__ __forceinline__ void tile_super_task(const FwdParams& p, typename Til
         it.hap = hap;
         it.out = st.aoff + (rd - st.r0) * st.nh + h;
         it.pad = 0;
-        tile_sweep<T, NR, FAST, ROWEPS>(p, tab, R, active, G, t, HL, bd.hap_off[hap], it,
-                                        rescue_base, bin);
+        done += tile_sweep<T, NR, FAST, ROWEPS, false, DS>(p, tab, R, active, G, t, HL,
+                                                          bd.hap_off[hap], it, rescue_base, bin);
     }
+    return done;
 }
 
 // Lane i < bs.n of a wave describes bin i of a BinSet: its id, lanes per
@@ -779,15 +941,17 @@ __device__ __forceinline__ int next_task(int32_t* ctr)
 // up to 256 VGPRs and run two.
 // The range-extended instances (a smaller share of the time, and a few more
 // live values in setup) are not held to three.
-template <typename T, int NR, bool FAST, bool RE>
+// The scaled instances are held to two where the register file allows no
+// more (the 20-row SAFE tile would otherwise take AGPRs and run one).
+template <typename T, int NR, bool FAST, bool RE, bool DS>
 constexpr int tile_waves_per_simd()
 {
-    return (sizeof(T) == 4 && !RE && (NR == 12 || (NR == 13 && FAST))) ? 3 : 1;
+    return (sizeof(T) == 4 && !RE && (NR == 12 || (NR == 13 && FAST))) ? 3 : DS ? 2 : 1;
 }
 
-template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false>
+template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false, bool DS = false>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE>(), 8))) void k_fwd_tile_set(
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE, DS>(), 8))) void k_fwd_tile_set(
     FwdParams p, BinSet bs)
 {
     static_assert(NR >= 4 && NR <= 32, "rows per lane");
@@ -796,6 +960,7 @@ __attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE>(), 8))) 
     const int total = set_lanes(bs, 0, sl);
     if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
         return;
+    int done = 0;  // DS: alignments finished by the scaling below the threshold
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
         // the task's bin, in scalar registers (uniform across the wave)
         const int k = set_find(sl, task);
@@ -810,14 +975,17 @@ __attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE>(), 8))) 
             __builtin_amdgcn_readfirstlane(bs.seg ? bs.seg[2 * rbin] : 0);
         if constexpr (!RE) {
             if (bs.st && bin < bs.item_bin0) {
-                tile_super_task<T, NR, FAST, ROWEPS>(p, tab, bs.st[begin + (task - t0)], G,
-                                                     rescue_base, rbin);
+                done += tile_super_task<T, NR, FAST, ROWEPS, DS>(p, tab, bs.st[begin + (task - t0)],
+                                                                 G, rescue_base, rbin);
                 continue;
             }
         }
-        tile_items_task<T, NR, FAST, ROWEPS, RE>(p, tab, bs.items, begin, count, G, task - t0,
-                                                 rescue_base, rbin);
+        done += tile_items_task<T, NR, FAST, ROWEPS, RE, DS>(p, tab, bs.items, begin, count, G,
+                                                             task - t0, rescue_base, rbin);
     }
+    if constexpr (DS)
+        if (done > 0 && threadIdx.x == 0)
+            atomicAdd(p.scaled_total, (unsigned long long)done);
 }
 
 // ---------------------------------------------------------------------------
@@ -1104,10 +1272,10 @@ static int resident_blocks_per_cu(K kernel)
     return n > 0 ? n : 4;
 }
 
-template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false>
+template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false, bool DS = false>
 static int set_blocks_per_cu()
 {
-    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST, ROWEPS, RE>);
+    static const int n = resident_blocks_per_cu(k_fwd_tile_set<T, NR, FAST, ROWEPS, RE, DS>);
     return n;
 }
 
@@ -1129,7 +1297,7 @@ static int grid_of(int grid, int per_cu)
     return cus * per_cu;
 }
 
-template <typename T, int NR, bool ROWEPS = false, bool RE = false>
+template <typename T, int NR, bool ROWEPS = false, bool RE = false, bool DS = false>
 static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs, int grid,
                                 hipStream_t st)
 {
@@ -1137,16 +1305,18 @@ static hipError_t launch_set_nr(bool fast, const FwdParams& p, const BinSet& bs,
         return hipErrorInvalidValue;
     if (RE && (bs.st || !bs.seg || !p.retry_items || !p.retry_counts || !p.rescue_exps))
         return hipErrorInvalidValue;
+    if (DS && (!p.scaled_total || !p.rescue_on))
+        return hipErrorInvalidValue;
     for (int i = 0; i < bs.n; ++i)
         if (bs.lanes[i] < 1 || bs.lanes[i] > 64)
             return hipErrorInvalidValue;
     if (fast) {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true, ROWEPS, RE>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true, ROWEPS, RE>())),
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, true, ROWEPS, RE, DS>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, true, ROWEPS, RE, DS>())),
                            dim3(64), 0, st, p, bs);
     } else {
-        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false, ROWEPS, RE>),
-                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false, ROWEPS, RE>())),
+        hipLaunchKernelGGL((k_fwd_tile_set<T, NR, false, ROWEPS, RE, DS>),
+                           dim3(grid_of(grid, set_blocks_per_cu<T, NR, false, ROWEPS, RE, DS>())),
                            dim3(64), 0, st, p, bs);
     }
     return hipGetLastError();
@@ -1163,6 +1333,19 @@ hipError_t launch_fwd_tile_set_re(int NR, bool fast, const FwdParams& p, const B
     case 16: return launch_set_nr<float, 16, false, true>(fast, p, bs, grid, st);
     case 19: return launch_set_nr<float, 19, false, true>(fast, p, bs, grid, st);
     case 20: return launch_set_nr<float, 20, false, true>(fast, p, bs, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fwd_tile_set_scaled(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                      int grid, hipStream_t st)
+{
+    switch (NR) {
+    case 12: return launch_set_nr<float, 12, false, false, true>(fast, p, bs, grid, st);
+    case 13: return launch_set_nr<float, 13, false, false, true>(fast, p, bs, grid, st);
+    case 16: return launch_set_nr<float, 16, false, false, true>(fast, p, bs, grid, st);
+    case 19: return launch_set_nr<float, 19, false, false, true>(fast, p, bs, grid, st);
+    case 20: return launch_set_nr<float, 20, false, false, true>(fast, p, bs, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1203,8 +1386,23 @@ hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const 
     }
 }
 
-int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps)
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps, bool scaled)
 {
+    if (scaled && !f64 && !roweps) {
+        switch (NR) {
+        case 12: return fast ? set_blocks_per_cu<float, 12, true, false, false, true>()
+                             : set_blocks_per_cu<float, 12, false, false, false, true>();
+        case 13: return fast ? set_blocks_per_cu<float, 13, true, false, false, true>()
+                             : set_blocks_per_cu<float, 13, false, false, false, true>();
+        case 16: return fast ? set_blocks_per_cu<float, 16, true, false, false, true>()
+                             : set_blocks_per_cu<float, 16, false, false, false, true>();
+        case 19: return fast ? set_blocks_per_cu<float, 19, true, false, false, true>()
+                             : set_blocks_per_cu<float, 19, false, false, false, true>();
+        case 20: return fast ? set_blocks_per_cu<float, 20, true, false, false, true>()
+                             : set_blocks_per_cu<float, 20, false, false, false, true>();
+        default: return 0;
+        }
+    }
     if (roweps) {
         if (!f64 && NR == kRowEpsRows32)
             return fast ? set_blocks_per_cu<float, kRowEpsRows32, true, true>()

@@ -166,6 +166,10 @@ struct FwdParams {
     uint8_t* rescue_exps;
     int32_t re_on;
     int32_t pad0;
+    // In-pass scaling (the scaled instances of the FP32 tiles): alignments
+    // whose sum fell below rescue_min and that the scaling finished are
+    // counted here (one atomic per wave and launch).
+    unsigned long long* scaled_total;
     float rescue_min;            // FP32 sum (reference units) threshold
     int32_t rescue_on;
     int32_t semantics;           // 0: reference GPU, 1: reference CPU, 2: per-position gcp
@@ -199,6 +203,42 @@ __host__ __device__ constexpr bool re_rows(int NR)
 {
     return NR == 12 || NR == 13 || NR == 16 || NR == 19 || NR == 20;
 }
+
+// In-pass scaling (the scaled FP32 tile instances; the same rows as the
+// range-extended pass).  Lane t of a group holds its values times 2^E_t
+// (E_t >= 0; E_0 = 0) and multiplies what it takes from lane t-1 by
+// 2^(E_t - E_{t-1}).  E_t starts from the insertion floor: every cell of a
+// row r >= 1 holds at least the path that enters row 1 anywhere and
+// inserts down to r, start * prod of the rows' insertion-extension
+// factors (the chain coefficient of each row), so lane t starts with its
+// last row's floor at 2^kDsFloor (E_t = kDsFloor - log2(start * floor
+// factors of its rows and those above), never below 0: lanes whose floor
+// stays above it are not scaled at all) and nothing it will hold can be
+// below its range; no lane starts or stays more than kDsSpan above the
+// lane over it.  Every kDsEvery steps a scaled lane whose incoming
+// values passed 2^kDsGuard scales down (never below E = 0) to put them at
+// 2^kDsGuardTo: a cell grows by the gap-open factor (2^13 at insertion
+// quality 40) in the step where a read's alignment reaches it and by the
+// gap-extension factor (2^3.3 at gcp 10) in the steps before, so over
+// kDsEvery steps about 2^63, which with the FAST form's D (about 2^13 above
+// its M) just fits above 2^kDsGuard (a growth past it overflows, which
+// fails acceptance); what the scale-down flushes lies at least 2^149 below
+// the lane's largest value.  Scaling by
+// powers of two is exact, so the sums are those of FP32 with an unbounded
+// exponent wherever no value leaves the normal range; lanes that never
+// leave E = 0 compute exactly what the unscaled instance computes.  A
+// result is finished in the pass when every lane up to the result lane
+// holds a running sum of at least 2^kReMinSum (finite where the lane is
+// scaled; an unscaled lane's sum may overflow harmlessly, as in the
+// unscaled instance), the result lane's sum is finite and neighbouring
+// exponents keep the transfer factor a normal float; the rest join the
+// rescue list.
+constexpr int kDsEvery = 16;
+constexpr int kDsFloor = -100;
+constexpr int kDsSpan = 127;  // largest E_t - E_{t-1}
+constexpr int kDsGuard = 55;
+constexpr float kDsGuardF = 0x1p55f;  // 2^kDsGuard
+constexpr int kDsGuardTo = 0;
 
 // Everything the binning kernels read and write (binning.hip).
 struct BinParams {
@@ -300,6 +340,10 @@ hipError_t launch_fwd_tile_set_f64(int NR, bool fast, const FwdParams& p, const 
 bool tile_re_supported(int NR);
 hipError_t launch_fwd_tile_set_re(int NR, bool fast, const FwdParams& p, const BinSet& bs,
                                   int grid, hipStream_t st);
+// The FP32 tile sets with in-pass scaling (rows per lane as for
+// tile_re_supported; not under per-position gcp).
+hipError_t launch_fwd_tile_set_scaled(int NR, bool fast, const FwdParams& p, const BinSet& bs,
+                                      int grid, hipStream_t st);
 hipError_t launch_fwd_wave_f32(const FwdParams& p, const BinSet& bs, const int32_t* hmax_ptr,
                                float* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
@@ -307,6 +351,6 @@ hipError_t launch_fwd_wave_f64(const FwdParams& p, const BinSet& bs, const int32
                                double* spill, int64_t spill_elems, int32_t* error_flag, int blocks,
                                hipStream_t st);
 // Resident blocks of 64 threads per CU for a tile-set instance.
-int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps = false);
+int tile_set_blocks_per_cu(int NR, bool fast, bool f64, bool roweps = false, bool scaled = false);
 
 }  // namespace phmm

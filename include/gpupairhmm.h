@@ -102,14 +102,20 @@ typedef struct phmm_options {
                                   (4, 6, 8 or 10; SAFE tiles at most 8); 0
                                   (default): chosen per bin from the
                                   measured FP64 tile rates */
-    int32_t range_rescue;      /* 1 (default): on the tuned table, results
-                                  below rescue_min_sum are first recomputed
-                                  in FP32 with a power-of-two scale per tile
-                                  lane, chosen from the first pass's per-lane
-                                  magnitudes; only what that pass cannot
-                                  accept goes to FP64.  0: every such result
-                                  is recomputed in FP64.  Not used under
-                                  PHMM_SEM_GCP or on the small-batch table. */
+    int32_t range_rescue;      /* 1 (default): the FP32 tile kernels keep
+                                  a power-of-two scale per tile lane, picked
+                                  during the sweep from the running sum of
+                                  the lane above, so sums below
+                                  rescue_min_sum are finished in the same
+                                  pass; what the scaling cannot vouch for
+                                  goes to FP64.  2: a second FP32 pass over
+                                  the results below rescue_min_sum, with
+                                  per-lane scales chosen from the first
+                                  pass's per-lane magnitudes (tuned table
+                                  only), then FP64.  0: every such result is
+                                  recomputed in FP64.  Tiles of 12, 13, 16,
+                                  19 and 20 rows per lane only; not under
+                                  PHMM_SEM_GCP. */
     int32_t reserved[2];
 } phmm_options;
 
@@ -167,7 +173,11 @@ typedef struct phmm_stats {
     double rescue_seconds;     /* event-timed rescue stage (if timed) */
     int64_t f64_alignments;    /* of the rescued alignments, those recomputed
                                   in FP64 */
-    int64_t reserved[4];
+    int64_t scaled_alignments; /* alignments whose FP32 sum fell below
+                                  rescue_min_sum and that the FP32 pass
+                                  finished with per-lane scaling
+                                  (range_rescue 1); not in `rescued` */
+    int64_t reserved[3];
 } phmm_stats;
 
 /* One row of the tile table (the reference's per-architecture

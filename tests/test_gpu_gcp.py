@@ -110,8 +110,8 @@ def test_gcp_device_path_and_rescue(lib, oracle):
 def test_gcp_equals_cpu_semantics_for_constant_gcp(lib):
     """With a constant gcp per read the per-row kernels compute what the
     first-gcp kernels compute, bit for bit (same coefficients, same order;
-    both with the FP64 rescue: the per-row kernels have no range-extended
-    pass)."""
+    both with the FP64 rescue: the per-row kernels have neither in-pass
+    scaling nor a range-extended pass)."""
     import gpupairhmm_amd as G
     b = G.HostBatch.synthetic(2, seed=55, num_records=6)
     say(f"case: gcp vs cpu semantics, constant gcp, {b.num_alignments} alignments")

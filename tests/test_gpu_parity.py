@@ -146,10 +146,12 @@ def test_plain_and_batched_identical(engine, plain_engine):
 
 
 def _same_but_rescued(a, b):
-    """Scores of one batch on two tile tables: bit-identical above the FP32
-    rescue floor; below it the range-extended pass may accept an alignment on
-    one table and hand it to FP64 on the other (both FP32-accurate)."""
-    hi = (a > FLOOR + 0.5) & (b > FLOOR + 0.5)
+    """Scores of one batch on two tile tables: bit-identical well above the
+    FP32 rescue floor; near and below it the lanes are scaled differently on
+    the two tables (contributions one flushes the other keeps) or a rescue
+    finishes an alignment on one table and hands it to FP64 on the other
+    (all FP32-accurate)."""
+    hi = (a > FLOOR + 10.0) & (b > FLOOR + 10.0)
     return (np.array_equal(a[hi].view(np.uint32), b[hi].view(np.uint32)) and
             _max_delta(a[~hi], b[~hi]) <= 5e-5)
 
@@ -304,10 +306,15 @@ def test_fp64_rescue(lib, oracle):
     ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
     say(f"case: FP64 rescue, oracle scores {ref}")
     assert np.all(ref < -80)
-    with G.Engine(fp64_rescue=True) as e:
+    with G.Engine(fp64_rescue=True, range_rescue=False) as e:
         got = e.score_batch(b)
         st = e.stats()
-    assert st.rescued == b.num_alignments
+    assert st.rescued == st.f64_alignments == b.num_alignments
+    assert _max_delta(got, ref) <= TOL
+    with G.Engine() as e:  # in-pass scaling first
+        got = e.score_batch(b)
+        st = e.stats()
+    assert st.rescued + st.scaled_alignments == b.num_alignments
     assert _max_delta(got, ref) <= TOL
     with G.Engine(fp64_rescue=False) as e:
         raw = e.score_batch(b)
@@ -335,7 +342,7 @@ def test_fp32_accurate_above_rescue_threshold(lib, oracle, config, records):
         got = e.score_batch(b)
         st = e.stats()
     assert _max_delta(got, ref) <= TOL
-    assert 0 < st.rescued < len(ref)
+    assert 0 < st.rescued + st.scaled_alignments < len(ref)
 
 
 @pytest.mark.parametrize("read_len,hap_len", [(800, 2000), (1300, 1500)])
@@ -348,12 +355,13 @@ def test_fp64_rescue_very_low_scores(lib, oracle, read_len, hap_len):
     ref = oracle.score_batch(b.oracle_arrays(), oracle.F64)
     say(f"case: very low scores (read {read_len}), oracle scores {ref}")
     assert np.all(ref < -300)
-    with G.Engine() as e:
-        got = e.score_batch(b)
-        st = e.stats()
-    assert st.rescued == b.num_alignments
-    assert np.all(np.isfinite(got))
-    assert _max_delta(got, ref) <= TOL
+    for rr in (False, True):  # FP64 alone; in-pass scaling first
+        with G.Engine(range_rescue=rr) as e:
+            got = e.score_batch(b)
+            st = e.stats()
+        assert st.rescued + st.scaled_alignments == b.num_alignments
+        assert np.all(np.isfinite(got))
+        assert _max_delta(got, ref) <= TOL
 
 
 def test_device_api_matches_host_api(engine):
