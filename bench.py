@@ -260,6 +260,11 @@ def bench_config(G, cfg, seed, records, steps, warmup, n_parity, time_kernels=Fa
            "alignments": batch.num_alignments, "cells": cells, "steps": steps,
            "gcups": round(cells * steps / dt / 1e9, 2),
            "ms_per_step": round(dt / steps * 1e3, 3),
+           # below the FP32 threshold: finished by the FP32 pass's per-lane
+           # scaling, or recomputed after it (of those, in FP64)
+           "below_threshold_fraction":
+               round((st.scaled_alignments + st.rescued) / max(steps * batch.num_alignments, 1), 5),
+           "scaled_fraction": round(st.scaled_alignments / max(steps * batch.num_alignments, 1), 5),
            "rescued_fraction": round(st.rescued / max(steps * batch.num_alignments, 1), 5),
            "f64_fraction": round(st.f64_alignments / max(steps * batch.num_alignments, 1), 5)}
     if time_kernels and st.kernel_seconds > 0:
@@ -459,6 +464,8 @@ def main():
                    "cells": cells, "mode": args.mode},
         "roofline": roofline,
         "parity": head["parity"],
+        "below_threshold_fraction": head["below_threshold_fraction"],
+        "scaled_fraction": head["scaled_fraction"],
         "rescued_fraction": head["rescued_fraction"],
         "f64_fraction": head["f64_fraction"],
         "rescue_stage_ms": head.get("rescue_stage_ms"),

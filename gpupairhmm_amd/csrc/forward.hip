@@ -478,26 +478,28 @@ __device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename Til
         return first ? (T)0 : (T)ldexpf(1.0f, d);
     };
     if constexpr (DS) {
-        // log2 of the insertion floor's factor over this lane's rows (the
-        // chain coefficient of each row; FAST row 0 has none), summed over
-        // the lanes above within the group
-        float lg = 0.0f;
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            const float e = FAST ? (float)R.c2[j] : (float)R.c3[j];
-            lg += e > 0.0f ? __log2f(e) : 0.0f;
-        }
-        float incl = lg;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const float y = __shfl_up(incl, o, 64);
-            if (t >= o)
-                incl += y;
+        // The insertion floor at the lane's last row R (the read's last
+        // row for the lanes past it): the product of the rows' chain
+        // coefficients from row 1 to R, which telescopes -- FAST: eps^R *
+        // delta_0 / delta_R / prod(alpha_0 .. alpha_{R-1}); SAFE: eps per
+        // row.  The alpha factors (each <= 1) are left out, which can only
+        // put the estimate below the true floor.  No lane exchange needed.
+        float fl = 0.0f;
+        if (active && C0 > 0 && L > 0) {
+            const int last = min((t + 1) * NR, L) - 1;
+            fl = __log2f((float)C0) + (float)last * __log2f((float)R.eps);
+            if (FAST && last > 0) {
+                const uint32_t* rw = p.rows + bd.read_off[it.read];
+                fl += __log2f(ph2pr_of(p.ph2pr, row_iq(rw[0]))) -
+                      __log2f(ph2pr_of(p.ph2pr, row_iq(rw[last])));
+            }
         }
         // the floor of the lane's last row at 2^kDsFloor
-        const float fl = (active && C0 > 0) ? __log2f((float)C0) + incl : 0.0f;
         const int e0 = (int)floorf((float)kDsFloor - fl);
-        ds_E = ds_span_cap(first || e0 < 0 ? 0 : e0);
+        int x = first || e0 < 0 ? 0 : e0;
+        if (__ballot(!first && x - ds_above(x) > kDsSpan))  // steep gap scores only
+            x = ds_span_cap(x);
+        ds_E = x;
         nf = ds_factor();
         ds_any = __ballot(ds_E > 0) != 0;
     }

@@ -209,13 +209,12 @@ __host__ __device__ constexpr bool re_rows(int NR)
 // (E_t >= 0; E_0 = 0) and multiplies what it takes from lane t-1 by
 // 2^(E_t - E_{t-1}).  E_t starts from the insertion floor: every cell of a
 // row r >= 1 holds at least the path that enters row 1 anywhere and
-// inserts down to r, start * prod of the rows' insertion-extension
-// factors (the chain coefficient of each row), so lane t starts with its
-// last row's floor at 2^kDsFloor (E_t = kDsFloor - log2(start * floor
-// factors of its rows and those above), never below 0: lanes whose floor
-// stays above it are not scaled at all) and nothing it will hold can be
-// below its range; no lane starts or stays more than kDsSpan above the
-// lane over it.  Every kDsEvery steps a scaled lane whose incoming
+// inserts down to r, start * prod of the rows' chain coefficients, which
+// telescopes (FAST: eps^r * delta_0 / delta_r / prod alpha; SAFE: eps^r),
+// so lane t starts with its last row's floor at 2^kDsFloor (E_t =
+// kDsFloor - log2 of that floor, never below 0: lanes whose floor stays
+// above it are not scaled at all) and nothing it will hold can be below its
+// range; no lane starts or stays more than kDsSpan above the lane over it.  Every kDsEvery steps a scaled lane whose incoming
 // values passed 2^kDsGuard scales down (never below E = 0) to put them at
 // 2^kDsGuardTo: a cell grows by the gap-open factor (2^13 at insertion
 // quality 40) in the step where a read's alignment reaches it and by the
