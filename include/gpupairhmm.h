@@ -89,15 +89,15 @@ typedef struct phmm_options {
                                   graph; 0 launches the kernels one by one */
     int32_t small_batch;       /* batches of at most this many alignments
                                   (either mode) use a one-template tile table
-                                  (16 rows per lane for every class) and
-                                  rescue in FP64 directly, so their latency
-                                  is one launch per stage rather than one per
-                                  tile template.  Scores are bit-identical
-                                  across tables above the rescue threshold;
-                                  below it the tuned table's range-extended
-                                  results agree with FP64 to FP32 accuracy
-                                  (within 5e-5 log10).  0: always the tuned
-                                  table.  Default 1 << 18. */
+                                  (16 rows per lane for every class; under
+                                  range_rescue 2 they rescue in FP64
+                                  directly), so their latency is one launch
+                                  per stage rather than one per tile
+                                  template.  Scores are bit-identical across
+                                  tables well above the rescue threshold;
+                                  near and below it they agree to FP32
+                                  accuracy (within 5e-5 log10).  0: always
+                                  the tuned table.  Default 1 << 18. */
     int32_t f64_rows;          /* rows per lane of every FP64 rescue tile
                                   (4, 6, 8 or 10; SAFE tiles at most 8); 0
                                   (default): chosen per bin from the
