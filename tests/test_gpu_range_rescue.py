@@ -206,3 +206,46 @@ def test_small_batches_rescue(lib):
         inpass = e.stats()
     assert inpass.scaled_alignments > 0
     assert inpass.f64_alignments == inpass.rescued <= 0.05 * small.rescued
+
+
+@pytest.mark.parametrize("gcp,bq", [(40, (20, 41)), (10, (2, 8)), (25, (2, 41))])
+def test_in_pass_scaling_extreme_qualities(lib, oracle, gcp, bq):
+    """Gap-continuation quality 40 (eps = 1e-4: the insertion floor falls
+    about 13 bits per row, so a 19-row lane spans more than the transfer
+    factor can, and the span cap pulls the lanes below along) and base
+    qualities down to 2 (emission values near 1/3 and 2/3): reads related and
+    unrelated to their haps, every result within FP32 accuracy of the FP64
+    oracle, whatever the scaling finishes and whatever it hands to FP64."""
+    import gpupairhmm_amd as G
+    rng = np.random.default_rng(31 + gcp)
+    rs = lambda n: "".join(rng.choice(list("ACGT"), n))
+    g = chr(33 + gcp)
+    recs = []
+    for L in (90, 150, 260):
+        hap = rs(L + 120)
+        reads = []
+        for k in range(6):
+            if k % 2:
+                body = rs(L)  # unrelated
+            else:
+                start = int(rng.integers(0, 100))
+                body = list(hap[start:start + L])
+                for i in rng.choice(L, L // 30, replace=False):
+                    body[i] = "ACGT"[("ACGT".index(body[i]) + 1) % 4]
+                body = "".join(body)
+            q = "".join(chr(33 + int(x)) for x in rng.integers(bq[0], bq[1], L))
+            reads.append(" ".join([body, q, "I" * L, "I" * L, g * L]))
+        recs.append("\n".join([f"{len(reads)} 2"] + reads + [hap, rs(L + 60)]))
+    b = G.HostBatch.parse("\n".join(recs) + "\n")
+    say(f"case: extreme qualities gcp {gcp} bq {bq}, {b.num_alignments} alignments")
+    # the GPU semantics fix eps at 0.1; the CPU semantics take it from the
+    # read's gcp quality
+    for sem, osem in ((G.SEM_GPU, oracle.SEM_GPU), (G.SEM_CPU, oracle.SEM_CPU)):
+        want = oracle.score_batch(b.oracle_arrays(), oracle.F64, semantics=osem)
+        for sb in (0, None):
+            with G.Engine(semantics=sem, small_batch=sb) as e:
+                got = e.score_batch(b)
+                st = e.stats()
+            print(f"semantics {sem} small_batch {sb}: scaled {st.scaled_alignments} "
+                  f"rescued {st.rescued} max |d| {_max_delta(got, want):.2e}")
+            assert _max_delta(got, want) <= FP32_TOL
