@@ -9,15 +9,18 @@ holds a read (gpuworker.cuh:243-273).  This tool measures instead, on the GPU:
    synthetic records of reads of length B, for both arithmetic forms (FAST
    reads: constant indel qualities; SAFE reads: indel qualities that jump
    between rows) and several hap lengths, with the engine's FP32 tile kernels
-   alone (plain mode, event-timed forward stage).
+   (the in-pass scaled instances, 12 to 20 rows per lane, as a default
+   engine runs them; plain mode, the event-timed forward and rescue stages).
 2. Class boundaries.  From those rates it fits a slot-rate model per rows
    per lane and form (cells per second of a full tile, scaled by the lanes a
    wave leaves idle and the hap's fill steps) and chooses up to 32 class
    bounds by dynamic programming, minimising the modelled time of a
    read-length distribution -- by default the mix of the benchmark
    workloads -- where every alignment pays for its class's whole tile.
-3. The table.  Each chosen class is measured again at its bound and takes
-   the fastest candidate (FAST and SAFE time weighted by the FAST share).
+3. The table.  Each chosen class is measured again at its bound, against a
+   hap 100 bases longer than its reads and a long one, and takes the
+   candidate with the least time per cell (FAST and SAFE weighted by the
+   FAST share, the two haps alike).
 4. FP64 tile rates.  For rows per lane 4, 6, 8, 10 and both forms it times
    the FP64 rescue stage with every alignment sent to it; the engine picks
    each bin's FP64 tile from these rates.
@@ -40,7 +43,12 @@ import time
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-ROWS = (4, 8, 12, 13, 16, 19, 20, 24, 28, 32)
+# rows per lane of the FP32 tiles with an in-pass scaled instance: the default
+# engine scales its FP32 tiles in the pass and runs the other row counts
+# (4, 8, 24, 28, 32) unscaled, handing each alignment that falls below
+# FLT_MIN to the FP64 rescue -- a cost the probe batches (no alignment below
+# FLT_MIN) cannot charge, so those rows are not candidates
+ROWS = (12, 13, 16, 19, 20)
 ROWS64 = (4, 6, 8, 10)
 # probe classes of the rate measurements; 103 and 151 hold reads of 101 and
 # 150-151 bases (common sequencer lengths) in a tile with at most one spare row
@@ -61,14 +69,27 @@ def candidates(bound: int, rows=ROWS):
     return out
 
 
+def probe_alignments(read_len: int) -> int:
+    """Alignments a probe batch needs so that the persistent grid runs many
+    rounds of tasks whatever the hap length: a batch that fills the device
+    only once or twice times the fill, the drain and one task's latency
+    rather than the tile's throughput.  Eight rounds of the device's waves
+    (256 CUs x 4 SIMDs x 3 waves), each wave holding as many alignments as
+    the fewest-lane 20-row candidate packs, times the 4 haps a read-stationary
+    task sweeps."""
+    per_wave = 64 // -(-(read_len + 1) // 20)
+    return 8 * (256 * 4 * 3) * 4 * per_wave
+
+
 def make_batch(G, read_len: int, hap_len: int, cells_target: float, seed: int, safe=False):
     """Records of 32 reads x 4 haps, all reads of one length, drawn from the
-    record's first hap with 1% errors.  safe: insertion qualities that jump
-    between rows, so no read takes the rescaled (FAST) form."""
+    record's first hap with 1% errors, at least probe_alignments(read_len)
+    alignments.  safe: insertion qualities that jump between rows, so no
+    read takes the rescaled (FAST) form."""
     rng = np.random.default_rng(seed)
-    per_rec = 32 * 4 * read_len * hap_len
-    n_rec = max(4, int(cells_target // per_rec))
     R, H = 32, 4
+    per_rec = R * H * read_len * hap_len
+    n_rec = max(4, int(cells_target // per_rec), -(-probe_alignments(read_len) // (R * H)))
     pad = lambda L: L + (4 - L % 4)
     rl = np.full(n_rec * R, read_len, dtype=np.int32)
     hl = np.full(n_rec * H, hap_len, dtype=np.int32)
@@ -109,8 +130,11 @@ def make_batch(G, read_len: int, hap_len: int, cells_target: float, seed: int, s
 
 
 def time_config(G, eng, dbatch, d_scores, cfg, bound, reps, stage="fwd"):
-    """Best event-timed duration of one stage over `reps` calls with the
-    one-class table (lanes, rows, bound)."""
+    """Best event-timed duration over `reps` calls with the one-class table
+    (lanes, rows, bound): stage "fwd" is the FP32 stage plus the rescue stage
+    (a tile without a scaled instance hands what falls below the FP32
+    threshold to FP64, which a forward-only time would not charge it);
+    "rescue" the rescue stage alone."""
     eng.set_tiles([(cfg[0], cfg[1], bound)])
     eng.score_device(dbatch, d_scores)  # warm-up
     eng.synchronize()
@@ -120,7 +144,8 @@ def time_config(G, eng, dbatch, d_scores, cfg, bound, reps, stage="fwd"):
         eng.score_device(dbatch, d_scores)
         eng.synchronize()
         st = eng.stats()
-        best = min(best, st.kernel_seconds if stage == "fwd" else st.rescue_seconds)
+        best = min(best, st.kernel_seconds + st.rescue_seconds if stage == "fwd"
+                   else st.rescue_seconds)
     return best
 
 
@@ -145,6 +170,13 @@ def measure_classes(G, eng, bounds, hap_lens, cells, reps, forms=("fast", "safe"
             log(f"len<={bound:5d} {form}: best {best['lanes']:2d}x{best['rows']:2d} "
                 f"{best['gcups']:8.1f} GCUPS (hap {best['hap_len']})")
     return out
+
+
+def final_haps(bound: int):
+    """Hap lengths a chosen class is measured against: one 100 bases longer
+    than its reads (the mixed-length workload's median extra) and one of
+    2000 or more (long active regions)."""
+    return (bound + 100, max(2000, 4 * bound))
 
 
 def lane_eff(g):
@@ -262,8 +294,10 @@ def measure_f64(G, cells, reps, log=print):
             if form == "safe" and nr > 8:
                 continue  # 10-row SAFE FP64 tiles need more than 256 VGPRs
             g = -(-152 // nr)
+            # range_rescue=0: every alignment goes to FP64 (in-pass scaling
+            # would finish them in the FP32 pass)
             with G.Engine(mode=G.MODE_PLAIN, time_kernels=True, rescue_min_sum=float("inf"),
-                          f64_rows=nr) as e2:
+                          range_rescue=0, f64_rows=nr) as e2:
                 db = e2.upload(batch)
                 d = e2.alloc(4 * batch.num_alignments)
                 t = time_config(G, e2, db, d, (8, 19), 151, reps, stage="rescue")
@@ -314,7 +348,9 @@ def main(argv=None):
     hap_lens = HAP_LENS[:1] if args.quick else HAP_LENS
     cells = args.cells / 4 if args.quick else args.cells
     t0 = time.time()
-    with G.Engine(mode=G.MODE_PLAIN, time_kernels=True, fp64_rescue=False) as eng:
+    # the FP32 kernels that run by default: the scaled instances (in-pass
+    # scaling needs the rescue on)
+    with G.Engine(mode=G.MODE_PLAIN, time_kernels=True) as eng:
         meas = measure_classes(G, eng, probe, hap_lens, cells, args.reps)
         rates = fit_rates(meas)
         print("slot rates (Gcells/s):",
@@ -322,19 +358,22 @@ def main(argv=None):
         p = workload_lengths()
         bounds, model = choose_bounds(rates, p, k=args.classes)
         print("model bounds:", bounds, flush=True)
-        # measure each chosen class at its bound (hap 300) and keep the fastest
-        final = measure_classes(G, eng, [b for b, _, _ in bounds], (300,), cells, args.reps)
+        # measure each chosen class at its bound, against a hap a little
+        # longer than its reads and a long one, and keep the candidate with
+        # the least time per cell over both forms and both haps
+        final = []
+        for b, _, _ in bounds:
+            final += measure_classes(G, eng, [b], final_haps(b), cells, args.reps)
         table = []
         for b, _, _ in bounds:
             best, bt = None, float("inf")
             for g, r in candidates(b):
-                tf = [m["seconds"] for m in final if m["bound"] == b and m["lanes"] == g and
-                      m["rows"] == r and m["form"] == "fast"]
-                ts = [m["seconds"] for m in final if m["bound"] == b and m["lanes"] == g and
-                      m["rows"] == r and m["form"] == "safe"]
-                if not tf or not ts:
+                per_cell = {f: [1 / m["gcups"] for m in final if m["bound"] == b and
+                                m["lanes"] == g and m["rows"] == r and m["form"] == f]
+                            for f in ("fast", "safe")}
+                if not per_cell["fast"] or not per_cell["safe"]:
                     continue
-                t = FAST_SHARE * tf[0] + (1 - FAST_SHARE) * ts[0]
+                t = FAST_SHARE * sum(per_cell["fast"]) + (1 - FAST_SHARE) * sum(per_cell["safe"])
                 if t < bt:
                     best, bt = (g, r), t
             table.append((best[0], best[1], b))
