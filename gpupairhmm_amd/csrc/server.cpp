@@ -5,9 +5,13 @@
 // arena -- the HBM budget less the device memory the server's launch
 // contexts hold (engine scratch, merge buffer, scores), counted at creation
 // from the real allocations -- and the in-flight limit; otherwise submit
-// returns PHMM_EBUSY.  An admitted request is staged once through pinned
-// memory into its arena block and stays device-resident until its scores
-// are returned.
+// returns PHMM_EBUSY.  An admitted request is staged in pinned memory and
+// copied into its arena block, where it stays until its scores are
+// returned.  The copy is issued at once when no copy is in flight, else by
+// the next flush (at the latest when the worker merges the request), which
+// moves every staged request with one copy per run contiguous in staging
+// and arena: a burst of arrivals costs a few copy commands, not one each,
+// and a launch waits only for the copies of what was staged before it.
 //
 // Launches.  A worker thread merges the admitted requests that wait (in
 // arrival order, up to the per-launch caps) as soon as one of the server's
@@ -25,6 +29,7 @@
 // coefficients only, so a request's scores are the same alone and in any
 // mix.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -109,6 +114,7 @@ struct Request {
     int64_t rbytes = 0, hbytes = 0;
     RequestLayout lay{};
     size_t off = 0, bytes = 0;  // arena block
+    uint64_t seq = 0;           // submission order
     Clock::time_point t_submit;
 };
 
@@ -136,6 +142,8 @@ struct LaunchCtx {
 
 // Groups a launch may hold per request it may merge.
 constexpr int kGroupsPerRequest = 16;
+// Staged bytes that are copied without waiting for the flush in flight.
+constexpr size_t kFlushBytes = 8 << 20;
 
 }  // namespace
 }  // namespace phmm
@@ -158,10 +166,30 @@ struct phmm_server {
     phmm::PinBuf stage;
     phmm::Ring stage_ring;
     hipStream_t copy_stream = nullptr;
-    hipEvent_t copy_mark = nullptr;  // recorded after every staged copy
-    std::deque<std::pair<size_t, hipEvent_t>> stage_pending;  // staging blocks in flight
+    hipEvent_t copy_mark = nullptr;  // recorded after every flush of staged copies
+    // Staged requests whose copies are not issued yet (submission order):
+    // staging offset, arena offset, bytes.  A flush issues one copy per run
+    // contiguous in both rings, so a backlog costs a few copy commands, not
+    // one per request.
+    struct Staged {
+        size_t soff, aoff, bytes;
+    };
+    std::vector<Staged> unflushed;
+    size_t unflushed_bytes = 0;
+    uint64_t next_seq = 0;
+    std::atomic<uint64_t> flushed_seq{0};  // requests below it have their copies issued
+    struct Flight {
+        std::vector<size_t> soffs;  // staging blocks the copies read
+        hipEvent_t ev;
+    };
+    std::deque<Flight> stage_pending;  // flushes in flight
     std::vector<hipEvent_t> event_pool;
-    std::mutex submit_mu;  // staging and the copy stream, in submission order
+    std::mutex submit_mu;  // staging, the copy stream and the fields above
+    // flushes whose copies failed: [first, second) submission ranges whose
+    // requests fail with copy_rc instead of launching (under mu)
+    std::vector<std::pair<uint64_t, uint64_t>> copy_failed;
+    int copy_rc = PHMM_OK;
+    std::string copy_msg;
     int64_t fixed_bytes = 0;
 
     std::mutex mu;
@@ -182,6 +210,7 @@ struct phmm_server {
     void launch(LaunchCtx& c, std::vector<Request>&& reqs);
     void fail_requests(std::vector<Request>& reqs, int rc, const std::string& msg);
     void reclaim_stage(bool block_one);
+    void flush_copies();
     int check(hipError_t e, const char* what)
     {
         if (e == hipSuccess)
@@ -286,8 +315,8 @@ phmm_server::~phmm_server()
     }
     if (copy_stream)
         hipStreamSynchronize(copy_stream);
-    for (auto& pr : stage_pending)
-        event_pool.push_back(pr.second);
+    for (Flight& f : stage_pending)
+        event_pool.push_back(f.ev);
     for (hipEvent_t e : event_pool)
         hipEventDestroy(e);
     if (copy_mark)
@@ -300,22 +329,77 @@ phmm_server::~phmm_server()
         hipHostFree(stage.p);
 }
 
-// Returns staging blocks whose copies finished (waiting for the oldest one
+// Returns staging blocks whose copies finished (waiting for the oldest flush
 // when block_one).  Caller holds submit_mu.
 void phmm_server::reclaim_stage(bool block_one)
 {
     while (!stage_pending.empty()) {
-        auto& pr = stage_pending.front();
-        const hipError_t q =
-            block_one ? hipEventSynchronize(pr.second) : hipEventQuery(pr.second);
+        Flight& f = stage_pending.front();
+        const hipError_t q = block_one ? hipEventSynchronize(f.ev) : hipEventQuery(f.ev);
         if (q == hipErrorNotReady)
             return;
         (void)hipGetLastError();
-        stage_ring.release(pr.first);
-        event_pool.push_back(pr.second);
+        for (size_t off : f.soffs)
+            stage_ring.release(off);
+        event_pool.push_back(f.ev);
         stage_pending.pop_front();
         block_one = false;
     }
+}
+
+// Issues the copies of every staged request into its arena block on the
+// copy stream, then records copy_mark (which launches wait for).  A failure
+// is kept in copy_rc: the worker fails those requests instead of launching
+// them.  Caller holds submit_mu, not mu.
+void phmm_server::flush_copies()
+{
+    if (unflushed.empty())
+        return;
+    hipSetDevice(eng->device);
+    hipEvent_t ev = nullptr;
+    if (!event_pool.empty()) {
+        ev = event_pool.back();
+        event_pool.pop_back();
+    }
+    Flight f;
+    f.soffs.reserve(unflushed.size());
+    int rc = ev ? PHMM_OK : check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    for (size_t i = 0; i < unflushed.size() && !rc;) {
+        size_t j = i + 1, bytes = unflushed[i].bytes;
+        while (j < unflushed.size() &&
+               unflushed[j].soff == unflushed[j - 1].soff + unflushed[j - 1].bytes &&
+               unflushed[j].aoff == unflushed[j - 1].aoff + unflushed[j - 1].bytes)
+            bytes += unflushed[j++].bytes;
+        rc = check(hipMemcpyAsync(static_cast<char*>(arena.p) + unflushed[i].aoff,
+                                  static_cast<const char*>(stage.p) + unflushed[i].soff, bytes,
+                                  hipMemcpyHostToDevice, copy_stream),
+                   "request copy");
+        i = j;
+    }
+    if (!rc && !(rc = check(hipEventRecord(ev, copy_stream), "event")))
+        rc = check(hipEventRecord(copy_mark, copy_stream), "event");
+    for (const Staged& u : unflushed)
+        f.soffs.push_back(u.soff);
+    unflushed.clear();
+    unflushed_bytes = 0;
+    if (rc) {
+        // nothing of this flush can be trusted to arrive: its staging is
+        // free again and its requests fail in the worker
+        (void)hipStreamSynchronize(copy_stream);
+        (void)hipGetLastError();
+        for (size_t off : f.soffs)
+            stage_ring.release(off);
+        if (ev)
+            event_pool.push_back(ev);
+        std::lock_guard<std::mutex> lk(mu);
+        copy_failed.push_back({flushed_seq.load(std::memory_order_relaxed), next_seq});
+        copy_rc = rc;
+        copy_msg = phmm::last_error();
+    } else {
+        f.ev = ev;
+        stage_pending.push_back(std::move(f));
+    }
+    flushed_seq.store(next_seq, std::memory_order_release);
 }
 
 void phmm_server::fail_requests(std::vector<Request>& reqs, int rc, const std::string& msg)
@@ -501,7 +585,7 @@ void phmm_server::run()
             progressed |= retire(c, false);
         std::vector<Request> batch;
         LaunchCtx* freec = nullptr;
-        bool any_busy = false;
+        bool any_busy = false, need_flush = false;
         {
             std::unique_lock<std::mutex> lk(mu);
             for (LaunchCtx& c : ctxs) {
@@ -511,7 +595,10 @@ void phmm_server::run()
             }
             if (stop && queue.empty() && !any_busy)
                 return;
-            if (freec && !queue.empty()) {
+            const uint64_t copied = flushed_seq.load(std::memory_order_acquire);
+            if (freec && !queue.empty() && queue.front().seq >= copied) {
+                need_flush = true;  // the next request's copy is not issued yet
+            } else if (freec && !queue.empty()) {
                 // merge waiting requests in arrival order, up to the caps:
                 // the latency cap unless the backlog says the server is
                 // overloaded
@@ -522,6 +609,8 @@ void phmm_server::run()
                 int64_t na = 0, bytes = 0, groups = 0;
                 while (!queue.empty() && (int)batch.size() < opt.max_batch_requests) {
                     const Request& r = queue.front();
+                    if (r.seq >= copied)
+                        break;
                     if (!batch.empty() && (na + r.nA > cap ||
                                            bytes + (int64_t)r.bytes > opt.max_batch_bytes ||
                                            groups + r.G > max_groups))
@@ -534,6 +623,20 @@ void phmm_server::run()
                     queue.pop_front();
                 }
                 st.queued = (int64_t)queue.size();
+                if (!copy_failed.empty()) {
+                    // requests of a flush whose copies failed are not launched
+                    std::vector<Request> bad, good;
+                    for (Request& r : batch) {
+                        bool failed = false;
+                        for (const auto& fr : copy_failed)
+                            failed |= r.seq >= fr.first && r.seq < fr.second;
+                        (failed ? bad : good).push_back(std::move(r));
+                    }
+                    if (!bad.empty())
+                        fail_requests(bad, copy_rc, copy_msg);
+                    batch = std::move(good);
+                    progressed = true;
+                }
             } else if (!progressed) {
                 if (!any_busy) {
                     cv.wait(lk, [&] { return stop || !queue.empty(); });
@@ -547,6 +650,12 @@ void phmm_server::run()
                     continue;
                 }
             }
+        }
+        if (need_flush) {
+            std::lock_guard<std::mutex> sl(submit_mu);
+            reclaim_stage(false);
+            flush_copies();
+            continue;
         }
         if (!batch.empty()) {
             launch(*freec, std::move(batch));
@@ -666,10 +775,14 @@ extern "C" int phmm_server_submit(phmm_server* s, const phmm_batch_view* req, ui
         s->st.inflight += 1;
         s->st.bytes_in_use = s->fixed_bytes + (int64_t)s->arena_ring.live;
     }
-    // stage the image in pinned memory, then one copy into the arena block
+    // stage the image in pinned memory; a flush copies it into the arena
     size_t soff = 0;
     s->reclaim_stage(false);
     while (!s->stage_ring.alloc(r.bytes, &soff)) {
+        if (!s->unflushed.empty()) {
+            s->flush_copies();  // staged blocks return only once copied
+            continue;
+        }
         if (s->stage_pending.empty()) {
             std::lock_guard<std::mutex> lk(s->mu);
             s->arena_ring.release(r.off);
@@ -696,29 +809,9 @@ extern "C" int phmm_server_submit(phmm_server* s, const phmm_batch_view* req, ui
     std::memcpy(P + L.nhaps, v.num_haps_per_group, 4 * (size_t)v.num_groups);
     std::memcpy(P + L.rprefix, v.reads_prefix, 4 * (size_t)(v.num_groups + 1));
     std::memcpy(P + L.hprefix, v.haps_prefix, 4 * (size_t)(v.num_groups + 1));
-    hipEvent_t ev = nullptr;
-    if (!s->event_pool.empty()) {
-        ev = s->event_pool.back();
-        s->event_pool.pop_back();
-    }
-    hipSetDevice(s->eng->device);
-    int rc = PHMM_OK;
-    if ((!ev && (rc = s->check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event"))) ||
-        (rc = s->check(hipMemcpyAsync(static_cast<char*>(s->arena.p) + r.off, P, (size_t)L.total,
-                                      hipMemcpyHostToDevice, s->copy_stream),
-                       "request copy")) ||
-        (rc = s->check(hipEventRecord(ev, s->copy_stream), "event")) ||
-        (rc = s->check(hipEventRecord(s->copy_mark, s->copy_stream), "event"))) {
-        if (ev)
-            s->event_pool.push_back(ev);
-        s->stage_ring.release(soff);
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->arena_ring.release(r.off);
-        s->st.inflight -= 1;
-        s->st.bytes_in_use = s->fixed_bytes + (int64_t)s->arena_ring.live;
-        return rc;
-    }
-    s->stage_pending.push_back({soff, ev});
+    r.seq = s->next_seq++;
+    s->unflushed.push_back({soff, r.off, r.bytes});
+    s->unflushed_bytes += r.bytes;
     r.t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -728,6 +821,12 @@ extern "C" int phmm_server_submit(phmm_server* s, const phmm_batch_view* req, ui
         s->st.queued = (int64_t)s->queue.size();
         s->st.max_queue_depth = std::max<int64_t>(s->st.max_queue_depth, s->st.queued);
     }
+    // copy at once when no flush is in flight; otherwise requests gather
+    // behind the one in flight (the worker flushes what it is about to
+    // merge) until kFlushBytes wait
+    s->reclaim_stage(false);
+    if (s->stage_pending.empty() || s->unflushed_bytes >= phmm::kFlushBytes)
+        s->flush_copies();
     s->cv.notify_all();
     return PHMM_OK;
 }
