@@ -455,6 +455,47 @@ def test_server_launches_overlap_and_merge(engine):
     assert st["bytes_in_use"] == st["scratch_bytes"]  # every arena block released
 
 
+def test_server_small_arena_stream(engine):
+    """A stream of requests through an arena that holds only a few of them:
+    submits are refused while it is full and admitted again as results come
+    back, arena blocks wrap around, and requests staged while a copy is in
+    flight are copied together (one copy per run contiguous in staging and
+    arena, split where the arena wraps).  Every result equals the request
+    scored alone."""
+    import gpupairhmm_amd as G
+    reqs = [G.HostBatch.synthetic(4, seed=800 + i, num_records=1) for i in range(48)]
+    alone = [engine.score_batch(r) for r in reqs]
+    caps = dict(max_batch_alignments=1 << 16, max_batch_bytes=8 << 20, max_batch_requests=8)
+    probe = G.Server(engine, **caps)
+    try:
+        fixed = probe.stats()["scratch_bytes"]
+    finally:
+        probe.close()
+    big = max(reqs, key=lambda b: b.num_alignments)
+    budget = fixed + 4 * (16 * big.num_alignments + (1 << 16))
+    srv = G.Server(engine, hbm_budget_bytes=budget, **caps)
+    say(f"case: serving stream of {len(reqs)} requests through a "
+        f"{(budget - fixed) >> 10} KiB arena")
+    got, nxt, refused = {}, 0, 0
+    try:
+        while len(got) < len(reqs):
+            while nxt < len(reqs) and srv.submit(reqs[nxt], nxt):
+                nxt += 1
+            if nxt < len(reqs):
+                refused += 1
+            res = srv.poll(timeout_ms=20000)
+            assert res is not None, "server timed out"
+            got[res[0]] = res[1]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for i in range(len(reqs)):
+        assert np.array_equal(alone[i].view(np.uint32), got[i].view(np.uint32)), i
+    assert refused > 0 and st["refused"] == refused  # the arena was full at times
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["bytes_in_use"] == st["scratch_bytes"]
+
+
 def test_server_budget_refusal(engine):
     """A budget that holds the launch contexts and one request but not two:
     the second request is refused with PHMM_EBUSY and admitted again once the
