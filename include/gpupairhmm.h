@@ -186,7 +186,11 @@ typedef struct phmm_stats {
  * lanes x `rows` rows per lane; lanes*rows must be >= max_read_len + 1. */
 typedef struct phmm_tile_config {
     int32_t lanes;         /* lanes per alignment, 1..64 */
-    int32_t rows;          /* rows per lane: 4, 8, 12, 13, 16, 19, 20, 24, 28 or 32 */
+    int32_t rows;          /* rows per lane: 4, 8, 12, 13, 16, 19, 20, 24, 28 or 32;
+                              only 12, 13, 16, 19 and 20 scale in the FP32
+                              pass (the others hand what falls below the
+                              FP32 floor to FP64), and 24-32 spill registers
+                              on gfx950: the tuned table uses 12-20 */
     int32_t max_read_len;  /* ascending across the table */
 } phmm_tile_config;
 
