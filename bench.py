@@ -416,8 +416,9 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(alignments shard by record with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (alignments "
+              "would shard by record, each GPU scoring whole records with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     import gpupairhmm_amd as G
