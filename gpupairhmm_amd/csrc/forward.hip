@@ -360,7 +360,12 @@ __device__ __forceinline__ void tile_setup(const FwdParams& p, typename TileTrai
 // Returns, for the scaled instances (DS), the number of the wave's
 // alignments that fell below the rescue threshold and were finished by the
 // scaling (uniform across the wave); 0 otherwise.
-template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false>
+// GE: the reference GPU semantics, whose gap continuation is the constant
+// kEpsGpu for every read: the D chain's multiplier is then wave-uniform and
+// sits in a scalar register, so those FMAs read two vector operands, not
+// three (about 1.4% of the sweep's time on gfx950).
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false,
+          bool GE = false>
 __device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename TileTraits<T>::V* tab,
                                           const TileRead<T, NR, ROWEPS>& R, bool active, int G,
                                           int t, int HL, int hoff, const WorkItem& it,
@@ -550,6 +555,14 @@ __device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename Til
             lamS[0] = stab[__umul24(letters & 3u, 64)];
     }
 
+    // GE: every lane's eps is kEpsGpu, held in a scalar register (opaque to
+    // the compiler, which would otherwise keep the constant in a vector one)
+    T eps_ge = (T)0;
+    if constexpr (GE) {
+        int e = __builtin_bit_cast(int, kEpsGpu);
+        asm volatile("" : "+s"(e));
+        eps_ge = (T)__builtin_bit_cast(float, e);
+    }
     auto sweep4 = [&](int s4, bool bounded) {
         // letters of steps s4+1 .. s4+4: the current window and the next one
         const uint32_t next = __builtin_amdgcn_alignbyte(wnx, whi, shift);
@@ -582,7 +595,7 @@ __device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename Til
                     chain = fma(R.c2[j], mp, R.c3[j] * chain);
                 }
                 M[j] = lvk * t1;
-                D[j] = fma((d_eps<T, NR, FAST, ROWEPS>(R, j)), dov, mo);
+                D[j] = fma(GE ? eps_ge : d_eps<T, NR, FAST, ROWEPS>(R, j), dov, mo);
                 mp = mo;
                 dp = dov;
             };
@@ -831,7 +844,8 @@ __device__ __forceinline__ int tile_sweep(const FwdParams& p, const typename Til
 
 // A task of 64/G consecutive alignments of an item list (the FP64 recompute
 // of rescue lists): every group sets up its own read and sweeps its own hap.
-template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false>
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE = false, bool DS = false,
+          bool GE = false>
 __device__ __forceinline__ int tile_items_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                 const WorkItem* items, int begin, int count, int G,
                                                 int task, int rescue_base, int bin)
@@ -857,15 +871,15 @@ __device__ __forceinline__ int tile_items_task(const FwdParams& p, typename Tile
     tile_setup<T, NR, FAST, ROWEPS>(p, tab, active, it.read, t, R);
     const int HL = active ? bd.hap_len[it.hap] : 0;
     const int hoff = active ? bd.hap_off[it.hap] : 0;
-    return tile_sweep<T, NR, FAST, ROWEPS, RE, DS>(p, tab, R, active, G, t, HL, hoff, it,
-                                                   rescue_base, bin);
+    return tile_sweep<T, NR, FAST, ROWEPS, RE, DS, GE>(p, tab, R, active, G, t, HL, hoff, it,
+                                                       rescue_base, bin);
 }
 
 // A super-task: the groups set up their reads once and sweep the segment's
 // haps in turn (all groups share each hap, so every sweep is uniform across
 // the wave).  Empty haps were scored -inf by the scatter; haps holding a
 // byte other than A/C/G/T belong to the exact bin.
-template <typename T, int NR, bool FAST, bool ROWEPS, bool DS = false>
+template <typename T, int NR, bool FAST, bool ROWEPS, bool DS = false, bool GE = false>
 __device__ __forceinline__ int tile_super_task(const FwdParams& p, typename TileTraits<T>::V* tab,
                                                const SuperTask& st, int G, int rescue_base, int bin)
 {
@@ -889,8 +903,9 @@ __device__ __forceinline__ int tile_super_task(const FwdParams& p, typename Tile
         it.hap = hap;
         it.out = st.aoff + (rd - st.r0) * st.nh + h;
         it.pad = 0;
-        done += tile_sweep<T, NR, FAST, ROWEPS, false, DS>(p, tab, R, active, G, t, HL,
-                                                          bd.hap_off[hap], it, rescue_base, bin);
+        done += tile_sweep<T, NR, FAST, ROWEPS, false, DS, GE>(p, tab, R, active, G, t, HL,
+                                                              bd.hap_off[hap], it, rescue_base,
+                                                              bin);
     }
     return done;
 }
@@ -965,18 +980,14 @@ constexpr int tile_waves_per_simd()
     return (sizeof(T) == 4 && !RE && (NR == 12 || (NR == 13 && FAST))) ? 3 : DS ? 2 : 1;
 }
 
-template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false, bool DS = false>
-__global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE, DS>(), 8))) void k_fwd_tile_set(
-    FwdParams p, BinSet bs)
+// The tasks of one launch, taken in order from the launch's counter.
+// Returns the alignments the scaling finished below the threshold (DS).
+template <typename T, int NR, bool FAST, bool ROWEPS, bool RE, bool DS, bool GE>
+__device__ __forceinline__ int tile_set_tasks(const FwdParams& p, const BinSet& bs,
+                                              typename TileTraits<T>::V* tab, const SetLane& sl,
+                                              int total)
 {
-    static_assert(NR >= 4 && NR <= 32, "rows per lane");
-    __shared__ typename TileTraits<T>::V tab[TabShape<T, NR>::kWords];
-    SetLane sl;
-    const int total = set_lanes(bs, 0, sl);
-    if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
-        return;
-    int done = 0;  // DS: alignments finished by the scaling below the threshold
+    int done = 0;
     for (int task = next_task(bs.next_task); task < total; task = next_task(bs.next_task)) {
         // the task's bin, in scalar registers (uniform across the wave)
         const int k = set_find(sl, task);
@@ -991,14 +1002,36 @@ __attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE, DS>(), 8
             __builtin_amdgcn_readfirstlane(bs.seg ? bs.seg[2 * rbin] : 0);
         if constexpr (!RE) {
             if (bs.st && bin < bs.item_bin0) {
-                done += tile_super_task<T, NR, FAST, ROWEPS, DS>(p, tab, bs.st[begin + (task - t0)],
-                                                                 G, rescue_base, rbin);
+                done += tile_super_task<T, NR, FAST, ROWEPS, DS, GE>(
+                    p, tab, bs.st[begin + (task - t0)], G, rescue_base, rbin);
                 continue;
             }
         }
-        done += tile_items_task<T, NR, FAST, ROWEPS, RE, DS>(p, tab, bs.items, begin, count, G,
-                                                             task - t0, rescue_base, rbin);
+        done += tile_items_task<T, NR, FAST, ROWEPS, RE, DS, GE>(p, tab, bs.items, begin, count,
+                                                                 G, task - t0, rescue_base, rbin);
     }
+    return done;
+}
+
+template <typename T, int NR, bool FAST, bool ROWEPS = false, bool RE = false, bool DS = false>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_waves_per_simd<T, NR, FAST, RE, DS>(), 8))) void k_fwd_tile_set(
+    FwdParams p, BinSet bs)
+{
+    static_assert(NR >= 4 && NR <= 32, "rows per lane");
+    __shared__ typename TileTraits<T>::V tab[TabShape<T, NR>::kWords];
+    SetLane sl;
+    const int total = set_lanes(bs, 0, sl);
+    if (total == 0)  // empty bins (common in a persistent launch): no atomic traffic
+        return;
+    // DS: alignments finished by the scaling below the threshold.  The
+    // scaled instances (the default FP32 pass) carry a copy of the sweep
+    // for the reference GPU semantics, whose eps is one constant.
+    int done;
+    if (DS && !ROWEPS && p.semantics == 0)
+        done = tile_set_tasks<T, NR, FAST, ROWEPS, RE, DS, DS && !ROWEPS>(p, bs, tab, sl, total);
+    else
+        done = tile_set_tasks<T, NR, FAST, ROWEPS, RE, DS, false>(p, bs, tab, sl, total);
     if constexpr (DS)
         if (done > 0 && threadIdx.x == 0)
             atomicAdd(p.scaled_total, (unsigned long long)done);
